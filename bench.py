@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Throughput of the NFFT hot path on one MI355X.
+
+Workload (one "step" = one nfftf_trafo + one nfftf_adjoint through the C ABI,
+device-resident inputs):
+  3-D NFFT, FP32 spreading / FFT with FP64 phi_hut and coordinate reduction,
+  N = 128^3 Fourier coefficients (iid complex N(0,1)), sigma = 2 (n = 256^3),
+  Kaiser-Bessel m = 4, M = 2^24 nodes on a 3-D radial "kooshball" (16384
+  spokes x 1024 samples, directions uniform on the sphere, radii -1/2 + k/1024).
+Metric: nodes/s = M / (t_trafo + t_adjoint), with precompute (node binning)
+excluded and reported separately.
+
+Also reported:
+  roofline      the dominant kernel (B interpolation or B^T spreading): its
+                algorithmic HBM bytes per launch (n_total c + M d r + M c)
+                over its average duration from HIP events on the plan's stream
+                inside the timed region, against 8 TB/s
+  step_hbm      the whole step's algorithmic bytes (D + FFT + B both ways)
+                over the step time
+  cpu_baseline  the FP64 CPU oracle on a bounded sample of the same workload
+                (full-grid D + FFT, B on 2^18 of the nodes, scaled to M),
+                single-threaded on the host
+
+python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "nodes/s 3-D trafo+adjoint (GB/s vs roofline); CGNR time-to-1e-6 learned vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(N, n, M, d, c, r):
+    """Algorithmic HBM bytes of trafo and adjoint stages (D, F, B)."""
+    Nt = int(np.prod(N))
+    nt = int(np.prod(n))
+    D = Nt * c + nt * c
+    F = d * 2 * nt * c
+    B = nt * c + M * d * r + M * c
+    DT = 2 * Nt * c
+    return dict(D=D, F=F, B=B, DT=DT, trafo=D + F + B, adjoint=B + F + DT)
+
+
+def cpu_baseline(N, n, m, x, fh, f, sample):
+    """FP64 oracle on the host: full-grid D + FFT both ways, B and B^T on
+    `sample` nodes, B time scaled to all nodes."""
+    import oracle as O
+    M = x.shape[0]
+    idx = np.random.default_rng(5).choice(M, size=sample, replace=False)
+    xs = np.ascontiguousarray(x[idx])
+    t0 = time.perf_counter()
+    g = O.stage_D(N, n, m, fh)
+    g = O.fft(n, g, -1)
+    t1 = time.perf_counter()
+    O.stage_B(N, n, m, xs, g)
+    t2 = time.perf_counter()
+    gt = O.stage_BT(N, n, m, xs, f[idx])
+    t3 = time.perf_counter()
+    gt = O.fft(n, g, +1)
+    O.stage_DT(N, n, m, gt)
+    t4 = time.perf_counter()
+    fixed = (t1 - t0) + (t4 - t3)
+    per_node = ((t2 - t1) + (t3 - t2)) / sample
+    est = fixed + per_node * M
+    return dict(value=M / est, unit="nodes/s", cores=1, kind="port",
+                sample="oracle (FP64, 1 thread) on the same workload: full 256^3 D+FFT for trafo "
+                       "and adjoint, B and B^T on %d random nodes of the kooshball, B time scaled "
+                       "to M=%d; measured %.1f s of CPU work" % (sample, M, t4 - t0))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--spokes", type=int, default=16384)
+    ap.add_argument("--samples", type=int, default=1024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 18)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import nfft_amd as nf
+    from nfft_amd import workloads as W
+
+    N = [128, 128, 128]
+    n = [256, 256, 256]
+    m = 4
+    d = 3
+    log("generating kooshball nodes (%d x %d)" % (args.spokes, args.samples))
+    x = W.kooshball(args.spokes, args.samples, seed=3)
+    M = x.shape[0]
+    fh = W.complex_normal(int(np.prod(N)), seed=4)
+    f_in = W.complex_normal(M, seed=5)
+
+    flags = (nf.PRE_PHI_HUT | nf.PRE_PSI | nf.MALLOC_X | nf.MALLOC_F_HAT | nf.MALLOC_F
+             | nf.FFTW_INIT | nf.NFFT_SORT_NODES)
+    p = nf.Plan(N, M, n=n, m=m, flags=flags, precision="single")
+    p.x[:] = x
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    p.set_stream(stream.cuda_stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p.precompute()
+    p.synchronize()
+    precompute_ms = (time.perf_counter() - t0) * 1e3
+    log("precompute %.1f ms, %d work items" % (precompute_ms, p.num_subproblems()))
+
+    d_fh = torch.from_numpy(fh.astype(np.complex64)).to(dev)
+    d_f = torch.empty(M, dtype=torch.complex64, device=dev)
+    d_fin = torch.from_numpy(f_in.astype(np.complex64)).to(dev)
+    d_fh_out = torch.empty(int(np.prod(N)), dtype=torch.complex64, device=dev)
+
+    def step():
+        p.trafo_device(d_fh.data_ptr(), d_f.data_ptr())
+        p.adjoint_device(d_fin.data_ptr(), d_fh_out.data_ptr())
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log("warmup done")
+
+    # timed region: K steps, HIP events around every stage on the plan stream
+    p.timing_begin()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    t_wall = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall_s = time.perf_counter() - t_wall
+    st = p.timing_end()
+    ms_total = ev0.elapsed_time(ev1)
+    ms_step = ms_total / args.steps
+    log("timed %d steps: %.3f ms/step (wall %.3f ms/step)" % (args.steps, ms_step, wall_s * 1e3 / args.steps))
+
+    K = args.steps
+    stages_ms = {k: v * 1e3 / K for k, v in st.items() if not k.startswith("n_")}
+    ab = algorithmic_bytes(N, n, M, d, 8, 4)
+    b_trafo_ms = stages_ms["trafo_B"]
+    b_adj_ms = stages_ms["adjoint_B"]
+    if b_adj_ms >= b_trafo_ms:
+        dom_name, dom_ms = "B^T spreading (k_spread_tile, adjoint)", b_adj_ms
+    else:
+        dom_name, dom_ms = "B interpolation (k_interp_tile, trafo)", b_trafo_ms
+    achieved = ab["B"] / (dom_ms * 1e-3) / 1e9
+    step_bytes = ab["trafo"] + ab["adjoint"]
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle, bounded sample) ...")
+        cpu = cpu_baseline(N, n, m, x, fh, f_in, min(args.cpu_sample, M))
+        log("cpu baseline %.3g nodes/s" % cpu["value"])
+
+    out = {
+        "metric": METRIC,
+        "value": M / (ms_step * 1e-3),
+        "unit": "nodes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "3-D NFFT trafo+adjoint, FP32, N=128^3, sigma=2 (n=256^3), Kaiser-Bessel m=4, "
+                        "M=2^24 kooshball nodes (16384 spokes x 1024 samples), device-resident",
+            "N": N, "n": n, "m": m, "M": M, "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_ms": dom_ms,
+            "algorithmic_bytes_per_launch": ab["B"],
+        },
+        "step_hbm": {
+            "algorithmic_bytes": step_bytes,
+            "achieved_GBs": step_bytes / (ms_step * 1e-3) / 1e9,
+            "frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        },
+        "stages_ms": stages_ms,
+        "precompute_ms": precompute_ms,
+        "cpu_baseline": cpu,
+    }
+    p.finalize()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,72 @@
+/*
+ * nfft3_hip.h -- device-resident extensions of libnfft3_hip.so.
+ *
+ * None of these has a counterpart in the reference: NFFT3's trafo/adjoint
+ * read and write host arrays only.  They serve callers that keep f_hat and f
+ * in GPU memory (the benchmark, an iterative solver, a pipeline of GPU
+ * stages) and want no PCIe copy per call.  Pointers named *_dev are HIP
+ * device pointers holding interleaved complex values in the plan's precision
+ * (fftwf_complex layout for nfftf_, fftw_complex for nfft_).  All work is
+ * enqueued on the plan's stream; the calls return without waiting unless
+ * noted.
+ */
+#ifndef NFFT3_HIP_EXT_H
+#define NFFT3_HIP_EXT_H
+
+#include "nfft3.h"
+
+#ifdef __cplusplus
+extern "C"
+{
+#endif
+
+#define NFFT_HIP_DECLARE_EXT(P, PLAN)                                        \
+  /* f_dev <- A f_hat_dev (same math as P##trafo) */                          \
+  void P##trafo_device(PLAN *ths, const void *f_hat_dev, void *f_dev);       \
+  /* f_hat_dev <- A^H f_dev (same math as P##adjoint) */                      \
+  void P##adjoint_device(PLAN *ths, const void *f_dev, void *f_hat_dev);     \
+  /* Run the plan's work on this hipStream_t from now on (NULL: the plan's   \
+   * own stream). */                                                         \
+  void P##set_stream(PLAN *ths, void *hip_stream);                           \
+  void *P##get_stream(PLAN *ths);                                            \
+  /* Wait for all work of the plan. */                                       \
+  void P##synchronize(PLAN *ths);                                            \
+  /* With on != 0, every trafo/adjoint records HIP events around its D, FFT  \
+   * and B stages and, once the call's work has finished, stores their       \
+   * durations in seconds in MEASURE_TIME_t[0..2] (the reference's           \
+   * --enable-measure-time fields).  The *_device calls then wait for their  \
+   * work before returning. */                                               \
+  void P##set_timing(PLAN *ths, int on);                                     \
+  /* Deferred stage timing for throughput runs: between timing_begin and     \
+   * timing_end every trafo / adjoint records HIP events around its stages   \
+   * on the plan's stream without waiting.  timing_end waits and writes     \
+   * out[0..7] = number of trafo calls, number of adjoint calls, then the    \
+   * summed seconds of trafo D, trafo FFT, trafo B (one interpolation        \
+   * kernel), adjoint B (one spreading kernel), adjoint FFT, adjoint D. */   \
+  void P##timing_begin(PLAN *ths);                                           \
+  void P##timing_end(PLAN *ths, double *out);                                \
+  /* Number of B-stage work items (sub-problems) built by the last           \
+   * precompute, for diagnostics. */                                         \
+  long P##num_subproblems(PLAN *ths);
+
+NFFT_HIP_DECLARE_EXT(nfftf_, nfftf_plan)
+NFFT_HIP_DECLARE_EXT(nfft_, nfft_plan)
+
+/* CGNR on one of this library's plans keeps all solver vectors on the GPU.
+ * The sync level says what is copied back to the host plan fields after
+ * before_loop and after every step: 0 the scalars (alpha_iter, beta_iter,
+ * dot_*), 1 also f_hat_iter, 2 (default) also r_iter, z_hat_iter,
+ * p_hat_iter and v_iter -- the reference's full host state.  *_sync_complex
+ * copies everything once.  Both are no-ops for solvers on the host path. */
+#define NFFT_HIP_DECLARE_SOLVER_EXT(S, SPLAN)                                \
+  void S##set_sync_complex(SPLAN *ths, int level);                           \
+  void S##sync_complex(SPLAN *ths);
+
+NFFT_HIP_DECLARE_SOLVER_EXT(solverf_, solverf_plan_complex)
+NFFT_HIP_DECLARE_SOLVER_EXT(solver_, solver_plan_complex)
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* NFFT3_HIP_EXT_H */

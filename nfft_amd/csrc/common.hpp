@@ -1,0 +1,43 @@
+// Shared definitions of the MI355X NFFT library: complex vector types,
+// error handling, and the per-precision hooks into the C API.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace nfft_hip {
+
+// Complex value types: interleaved (re, im), the memory layout of
+// fftw_complex / fftwf_complex.
+template <typename R> struct ctype;
+template <> struct ctype<float> { using type = float2; };
+template <> struct ctype<double> { using type = double2; };
+template <typename R> using cplx = typename ctype<R>::type;
+
+// Formats a message and ends the process through the family's die hook
+// (nfft_die / nfftf_die: hook, then exit(EXIT_FAILURE)), the reference's
+// only fatal-error channel (kernel/util/malloc.c:58-64).
+[[noreturn]] void fatal_float(const char *fmt, ...);
+[[noreturn]] void fatal_double(const char *fmt, ...);
+
+template <typename R> struct family;
+template <> struct family<float> {
+  template <typename... A> [[noreturn]] static void fatal(const char *f, A... a) { fatal_float(f, a...); }
+};
+template <> struct family<double> {
+  template <typename... A> [[noreturn]] static void fatal(const char *f, A... a) { fatal_double(f, a...); }
+};
+
+#define NFFT_HIP_CHECK(R, expr)                                                   \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      ::nfft_hip::family<R>::fatal("%s failed (%s:%d): %s", #expr, __FILE__,      \
+                                   __LINE__, hipGetErrorString(e_));              \
+  } while (0)
+
+inline long ceil_div(long a, long b) { return (a + b - 1) / b; }
+
+}  // namespace nfft_hip

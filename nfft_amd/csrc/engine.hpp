@@ -1,0 +1,137 @@
+// Device-side state of one NFFT plan and the launchers of its kernels.
+//
+// One Engine<R> lives behind each nfft_plan / nfftf_plan (in the plan's
+// my_fftw_plan1 field).  Data layout in HBM:
+//   g        oversampled grid, n_total complex, row-major (last dim fastest),
+//            transformed in place (D -> FFT -> B and B^T -> FFT -> D^T)
+//   fhat, f  device images of the plan's f_hat (N_total) and f (M_total)
+//   cell     d x M ints, SoA: wrapped cell floor(n_t x_t) mod n_t of every node,
+//            in bin-sorted order
+//   frac     d x M reals, SoA: n_t x_t - floor(n_t x_t) in [0,1), same order
+//   perm     M ints: bin-sorted position -> node index of the caller's order
+//   subs     work items of the B kernels: (bin, first sorted node, count)
+#pragma once
+
+#include <array>
+#include <vector>
+
+#include "common.hpp"
+
+namespace nfft_hip {
+
+constexpr int kMaxDim = 3;
+
+// Geometry shared by the node-binning and B kernels.
+struct BinGeom {
+  int d;
+  int m;
+  int n[kMaxDim];   // oversampled grid lengths
+  int bs[kMaxDim];  // bin edge in cells
+  int nb[kMaxDim];  // bins per dimension
+  int E[kMaxDim];   // LDS tile edge: bs + 2m + 1
+  long M;
+};
+
+// Kaiser-Bessel window of one dimension, scaled by 1/phi(0) so that the B
+// weights stay O(1) (the scale is folded into the deconvolution factors).
+template <typename R> struct WinParams {
+  R b[kMaxDim];        // shape parameter pi (2 - 1/sigma_t)
+  R inv_den[kMaxDim];  // 1 / (1 - exp(-2 b m))
+};
+
+// One axis pass of the oversampled FFT: the grid viewed as
+// [outer][len][inner]; each line (outer o, inner i) is transformed.
+template <typename R> struct FftAxis {
+  int len;
+  long inner;
+  long nlines;
+  int lines_per_block;
+  int nstages;
+  int radix[20];
+  cplx<R> *twiddle;  // exp(-2 pi i k / len), k < len
+};
+
+template <typename R> struct Engine {
+  // geometry
+  int d = 0;
+  long N[kMaxDim] = {1, 1, 1};
+  long n[kMaxDim] = {1, 1, 1};
+  long N_total = 0, n_total = 0, M = 0;
+  int m = 0;
+  bool direct_only = false;  // some N_t <= m or n_t <= 2m+2: NDFT path
+
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  bool timing = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool deferred = false;  // timing_begin .. timing_end
+  std::vector<std::array<hipEvent_t, 4>> log_ev;
+  std::vector<int> log_kind;  // 0 trafo, 1 adjoint
+  size_t log_n = 0;
+
+  // device buffers
+  cplx<R> *g = nullptr;
+  cplx<R> *fhat = nullptr;
+  cplx<R> *f = nullptr;
+  R *xdev = nullptr;
+  R *phi_inv = nullptr;  // sum_t N_t scaled deconvolution factors
+  int *cell = nullptr;
+  R *frac = nullptr;
+  int *perm = nullptr;
+  int4 *subs = nullptr;
+  long nsub = 0;
+  long subs_cap = 0;
+  int *scratch_i = nullptr;  // binning scratch
+  long scratch_cap = 0;
+  bool nodes_ready = false;
+
+  BinGeom geom{};
+  WinParams<R> win{};
+  int max_per_sub = 1024;
+  std::vector<FftAxis<R>> axes;
+
+  void create(int d_, const long *N_, const long *n_, long M_, int m_);
+  void destroy();
+  void upload_nodes(const R *x_host);
+  void build_nodes();  // from xdev
+  void trafo(const cplx<R> *fhat_dev, cplx<R> *f_dev);
+  void adjoint(const cplx<R> *f_dev, cplx<R> *fhat_dev);
+  void trafo_direct(const cplx<R> *fhat_dev, cplx<R> *f_dev);
+  void adjoint_direct(const cplx<R> *f_dev, cplx<R> *fhat_dev);
+  void stage_times(double out[3]);
+  hipEvent_t *call_events(int kind);
+  void timing_begin();
+  void timing_end(double *out);
+};
+
+// Host arrays owned by the library (plan MALLOC_* arrays, solver vectors):
+// page-locked unless the family's malloc hook is set.
+template <typename R> void *host_array_alloc(size_t bytes);
+template <typename R> void host_array_free(void *p);
+// Engine behind a plan whose mv_trafo is this library's trafo, else nullptr.
+template <typename R> Engine<R> *engine_of_mv(void *mv);
+// Node data of such a plan, built as its trafo would build them.
+template <typename R> void ensure_nodes_of_mv(void *mv);
+
+// ---- launchers (explicitly instantiated for float and double) ----
+template <typename R>
+void launch_fill_zero(cplx<R> *p, long count, hipStream_t s);
+template <typename R>
+void launch_deconv_to_grid(const cplx<R> *fhat, cplx<R> *g, const R *phi_inv, int d,
+                           const long *N, const long *n, hipStream_t s);
+template <typename R>
+void launch_deconv_from_grid(const cplx<R> *g, cplx<R> *fhat, const R *phi_inv, int d,
+                             const long *N, const long *n, hipStream_t s);
+template <typename R>
+void launch_fft_axis(cplx<R> *data, const FftAxis<R> &ax, int sign, hipStream_t s);
+template <typename R>
+void launch_interp(const cplx<R> *g, cplx<R> *f, const Engine<R> &e, hipStream_t s);
+template <typename R>
+void launch_spread(const cplx<R> *f, cplx<R> *g, const Engine<R> &e, hipStream_t s);
+template <typename R>
+void launch_ndft(const R *x, const cplx<R> *in, cplx<R> *out, int d, const long *N, long M,
+                 int adjoint, hipStream_t s);
+template <typename R>
+void bin_nodes(Engine<R> &e);
+
+}  // namespace nfft_hip

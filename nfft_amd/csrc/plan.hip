@@ -1,0 +1,728 @@
+// The NFFT plan API of the C ABI (include/nfft3.h) for the nfftf_ (float)
+// and nfft_ (double) families, and the Engine that runs a plan on the GPU.
+//
+// Reference behaviour followed (kernel/nfft/nfft.c):
+//   init_help 5950-6046   geometry, sigma, b, MALLOC_* ownership, phi_hut
+//   init / init_guru / init_lin / init_1d,2d,3d 6048-6167   defaults: n_t =
+//       2 next_power_of_2(N_t), m = 4 (float) / 8 (double), flags
+//   precompute_one_psi 5938-5948, trafo 5655-5701, adjoint 5703-5749 (direct
+//       NDFT whenever some N_t <= m or n_t <= 2m+2), check 6169-6207,
+//       finalize 6209-6270
+// Host arrays stay authoritative: every trafo/adjoint copies f_hat / f at the
+// plan's current pointers (the solver swaps them between calls).  Node data
+// are taken from x at precompute time when a PRE_* psi flag is set, and from
+// the current x on every call otherwise, as the reference does.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/nfft3.h"
+#include "../../include/nfft3_hip.h"
+#include "engine.hpp"
+
+namespace nfft_hip {
+
+// ------------------------------------------------------------------ helpers
+// I0 by its power series sum_k ((x/2)^2)^k / (k!)^2, in long double.
+static long double bessel_i0(long double x) {
+  long double q = x * x / 4.0L, term = 1.0L, sum = 1.0L;
+  for (int k = 1; k < 4000; k++) {
+    term *= q / ((long double)k * (long double)k);
+    sum += term;
+    if (term < sum * 1e-21L) break;
+  }
+  return sum;
+}
+
+static const long double kPiL = 3.141592653589793238462643383279502884L;
+
+// PHI_HUT(n, k) = I0(m sqrt(b^2 - (2 pi k / n)^2))  (include/infft.h:208)
+static long double phi_hut(long n, long k, int m, long double b) {
+  long double w = 2.0L * kPiL * (long double)k / (long double)n;
+  return bessel_i0((long double)m * sqrtl(b * b - w * w));
+}
+
+// factor n into FFT radices, largest first; returns false if a prime > 13 remains
+static bool fft_radices(long n, std::vector<int> &out) {
+  out.clear();
+  while (n % 8 == 0 && n > 1) { out.push_back(8); n /= 8; }
+  while (n % 4 == 0 && n > 1) { out.push_back(4); n /= 4; }
+  while (n % 2 == 0 && n > 1) { out.push_back(2); n /= 2; }
+  for (int p : {3, 5, 7, 11, 13})
+    while (n % p == 0) { out.push_back(p); n /= p; }
+  return n == 1;
+}
+
+template <typename R>
+void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) {
+  d = d_;
+  M = M_;
+  m = m_;
+  N_total = 1;
+  n_total = 1;
+  for (int t = 0; t < d; t++) {
+    N[t] = N_[t];
+    n[t] = n_[t];
+    N_total *= N[t];
+    n_total *= n[t];
+    if (N[t] <= m || n[t] <= 2 * m + 2) direct_only = true;
+  }
+  // an oversampled grid shorter than N would wrap frequencies onto each other
+  for (int t = 0; t < d; t++)
+    if (N[t] < 1 || (!direct_only && n[t] < N[t]))
+      family<R>::fatal("nfft: need N[%d] >= 1 and n[%d] >= N[%d] (got N = %ld, n = %ld)", t, t, t,
+                       N[t], n[t]);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    family<R>::fatal("nfft: no HIP device available (this library runs on the GPU only)");
+  NFFT_HIP_CHECK(R, hipStreamCreate(&own_stream));
+  stream = own_stream;
+  for (auto &e : ev) NFFT_HIP_CHECK(R, hipEventCreate(&e));
+
+  NFFT_HIP_CHECK(R, hipMalloc(&fhat, sizeof(cplx<R>) * (size_t)(N_total > 0 ? N_total : 1)));
+  NFFT_HIP_CHECK(R, hipMalloc(&f, sizeof(cplx<R>) * (size_t)(M > 0 ? M : 1)));
+  NFFT_HIP_CHECK(R, hipMalloc(&xdev, sizeof(R) * (size_t)(M > 0 ? M * d : 1)));
+  if (direct_only) return;
+
+  if (m > 16) family<R>::fatal("nfft: window cut-off m = %d exceeds the supported 16", m);
+
+  NFFT_HIP_CHECK(R, hipMalloc(&g, sizeof(cplx<R>) * (size_t)n_total));
+  NFFT_HIP_CHECK(R, hipMalloc(&cell, sizeof(int) * (size_t)(M > 0 ? M * d : 1)));
+  NFFT_HIP_CHECK(R, hipMalloc(&frac, sizeof(R) * (size_t)(M > 0 ? M * d : 1)));
+  NFFT_HIP_CHECK(R, hipMalloc(&perm, sizeof(int) * (size_t)(M > 0 ? M : 1)));
+
+  // window parameters and scaled deconvolution factors
+  std::vector<R> pinv;
+  for (int t = 0; t < d; t++) {
+    long double sigma = (long double)n[t] / (long double)N[t];
+    long double b = kPiL * (2.0L - 1.0L / sigma);
+    long double peak = sinhl(b * m) / (kPiL * m);  // phi(0)
+    win.b[t] = (R)b;
+    win.inv_den[t] = (R)(1.0L / (-expm1l(-2.0L * b * m)));
+    for (long ks = 0; ks < N[t]; ks++) pinv.push_back((R)(peak / phi_hut(n[t], ks - N[t] / 2, m, b)));
+  }
+  NFFT_HIP_CHECK(R, hipMalloc(&phi_inv, sizeof(R) * pinv.size()));
+  NFFT_HIP_CHECK(R, hipMemcpy(phi_inv, pinv.data(), sizeof(R) * pinv.size(), hipMemcpyHostToDevice));
+
+  // bins: shrink the largest edge until the LDS tile fits the budget
+  const size_t budget = 48 * 1024;
+  int bs[kMaxDim] = {1, 1, 1};
+  const int init = d == 1 ? 4096 : d == 2 ? 64 : 16;
+  for (int t = 0; t < d; t++) bs[t] = (int)std::min<long>(n[t], init);
+  auto bytes = [&]() {
+    size_t s = sizeof(cplx<R>);
+    for (int t = 0; t < d; t++) s *= (size_t)(bs[t] + 2 * m + 1);
+    return s;
+  };
+  while (bytes() > budget) {
+    int tmax = 0;
+    for (int t = 1; t < d; t++)
+      if (bs[t] > bs[tmax]) tmax = t;
+    if (bs[tmax] == 1) break;
+    bs[tmax] = (bs[tmax] + 1) / 2;
+  }
+  geom.d = d;
+  geom.m = m;
+  geom.M = M;
+  for (int t = 0; t < kMaxDim; t++) {
+    geom.n[t] = t < d ? (int)n[t] : 1;
+    geom.bs[t] = t < d ? bs[t] : 1;
+    geom.nb[t] = t < d ? (int)ceil_div(n[t], bs[t]) : 1;
+    geom.E[t] = t < d ? bs[t] + 2 * m + 1 : 1;
+  }
+  max_per_sub = d == 3 ? 256 : 512;
+
+  // FFT axis passes
+  const int cap = sizeof(R) == 4 ? 4096 : 2048;  // elements per workgroup
+  long inner = n_total;
+  for (int t = 0; t < d; t++) {
+    FftAxis<R> ax{};
+    ax.len = (int)n[t];
+    inner /= n[t];
+    ax.inner = inner;
+    ax.nlines = n_total / n[t];
+    if (n[t] > cap)
+      family<R>::fatal("nfft: FFT length n[%d] = %ld exceeds the supported %d", t, n[t], cap);
+    std::vector<int> rad;
+    if (!fft_radices(n[t], rad))
+      family<R>::fatal("nfft: FFT length n[%d] = %ld has a prime factor above 13", t, n[t]);
+    ax.nstages = (int)rad.size();
+    for (size_t i = 0; i < rad.size(); i++) ax.radix[i] = rad[i];
+    ax.lines_per_block = std::max(1, cap / (int)n[t]);
+    std::vector<cplx<R>> tw(n[t]);
+    for (long k = 0; k < n[t]; k++) {
+      long double a = -2.0L * kPiL * (long double)k / (long double)n[t];
+      tw[k] = cplx<R>{(R)cosl(a), (R)sinl(a)};
+    }
+    NFFT_HIP_CHECK(R, hipMalloc(&ax.twiddle, sizeof(cplx<R>) * tw.size()));
+    NFFT_HIP_CHECK(R, hipMemcpy(ax.twiddle, tw.data(), sizeof(cplx<R>) * tw.size(),
+                                hipMemcpyHostToDevice));
+    axes.push_back(ax);
+  }
+}
+
+template <typename R>
+void Engine<R>::destroy() {
+  if (stream) hipStreamSynchronize(stream);
+  for (auto &ax : axes) hipFree(ax.twiddle);
+  axes.clear();
+  hipFree(g);
+  hipFree(fhat);
+  hipFree(f);
+  hipFree(xdev);
+  hipFree(phi_inv);
+  hipFree(cell);
+  hipFree(frac);
+  hipFree(perm);
+  hipFree(subs);
+  hipFree(scratch_i);
+  for (auto &e : ev)
+    if (e) hipEventDestroy(e);
+  for (auto &set : log_ev)
+    for (auto &e : set) hipEventDestroy(e);
+  if (own_stream) hipStreamDestroy(own_stream);
+}
+
+template <typename R>
+void Engine<R>::upload_nodes(const R *x_host) {
+  if (M == 0) return;
+  NFFT_HIP_CHECK(R, hipMemcpyAsync(xdev, x_host, sizeof(R) * (size_t)(M * d),
+                                   hipMemcpyHostToDevice, stream));
+}
+
+template <typename R>
+void Engine<R>::build_nodes() {
+  if (!direct_only) bin_nodes<R>(*this);
+  nodes_ready = true;
+}
+
+// events bracketing the stages of one call: the immediate set, a fresh set
+// from the deferred log, or none
+template <typename R>
+hipEvent_t *Engine<R>::call_events(int kind) {
+  if (deferred) {
+    if (log_n == log_ev.size()) {
+      std::array<hipEvent_t, 4> set;
+      for (auto &e : set) NFFT_HIP_CHECK(R, hipEventCreate(&e));
+      log_ev.push_back(set);
+      log_kind.push_back(0);
+    }
+    log_kind[log_n] = kind;
+    return log_ev[log_n++].data();
+  }
+  return timing ? ev : nullptr;
+}
+
+template <typename R>
+void Engine<R>::trafo(const cplx<R> *fhat_dev, cplx<R> *f_dev) {
+  hipEvent_t *E = call_events(0);
+  if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[0], stream));
+  launch_fill_zero<R>(g, n_total, stream);
+  launch_deconv_to_grid<R>(fhat_dev, g, phi_inv, d, N, n, stream);
+  if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[1], stream));
+  for (auto &ax : axes) launch_fft_axis<R>(g, ax, -1, stream);
+  if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[2], stream));
+  launch_interp<R>(g, f_dev, *this, stream);
+  if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[3], stream));
+  NFFT_HIP_CHECK(R, hipGetLastError());
+}
+
+template <typename R>
+void Engine<R>::adjoint(const cplx<R> *f_dev, cplx<R> *fhat_dev) {
+  hipEvent_t *E = call_events(1);
+  launch_fill_zero<R>(g, n_total, stream);
+  if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[0], stream));
+  launch_spread<R>(f_dev, g, *this, stream);
+  if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[1], stream));
+  for (auto &ax : axes) launch_fft_axis<R>(g, ax, +1, stream);
+  if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[2], stream));
+  launch_deconv_from_grid<R>(g, fhat_dev, phi_inv, d, N, n, stream);
+  if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[3], stream));
+  NFFT_HIP_CHECK(R, hipGetLastError());
+}
+
+template <typename R>
+void Engine<R>::timing_begin() {
+  deferred = true;
+  log_n = 0;
+}
+
+template <typename R>
+void Engine<R>::timing_end(double *out) {
+  for (int i = 0; i < 8; i++) out[i] = 0;
+  NFFT_HIP_CHECK(R, hipStreamSynchronize(stream));
+  for (size_t c = 0; c < log_n; c++) {
+    float ms[3];
+    for (int i = 0; i < 3; i++)
+      NFFT_HIP_CHECK(R, hipEventElapsedTime(&ms[i], log_ev[c][i], log_ev[c][i + 1]));
+    const int k = log_kind[c];
+    out[k] += 1;
+    for (int i = 0; i < 3; i++) out[2 + 3 * k + i] += ms[i] * 1e-3;
+  }
+  deferred = false;
+  log_n = 0;
+}
+
+template <typename R>
+void Engine<R>::trafo_direct(const cplx<R> *fhat_dev, cplx<R> *f_dev) {
+  launch_ndft<R>(xdev, fhat_dev, f_dev, d, N, M, 0, stream);
+  NFFT_HIP_CHECK(R, hipGetLastError());
+}
+
+template <typename R>
+void Engine<R>::adjoint_direct(const cplx<R> *f_dev, cplx<R> *fhat_dev) {
+  launch_ndft<R>(xdev, f_dev, fhat_dev, d, N, M, 1, stream);
+  NFFT_HIP_CHECK(R, hipGetLastError());
+}
+
+// stage durations of the last call, in seconds: [0] D, [1] FFT, [2] B
+template <typename R>
+void Engine<R>::stage_times(double out[3]) {
+  float ms[3] = {0, 0, 0};
+  NFFT_HIP_CHECK(R, hipEventSynchronize(ev[3]));
+  for (int i = 0; i < 3; i++) NFFT_HIP_CHECK(R, hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+  // trafo records D, FFT, B in that order; adjoint B, FFT, D -- the caller
+  // maps the intervals onto MEASURE_TIME_t
+  for (int i = 0; i < 3; i++) out[i] = ms[i] * 1e-3;
+}
+
+template struct Engine<float>;
+template struct Engine<double>;
+
+// ------------------------------------------------------------------ plan API
+template <typename R> struct Fam;
+template <> struct Fam<float> {
+  static void *alloc(size_t n) { return nfftf_malloc(n); }
+  static void release(void *p) { nfftf_free(p); }
+  static bool hooked() { return nfftf_malloc_hook != nullptr; }
+  static constexpr int default_m = 4;
+};
+template <> struct Fam<double> {
+  static void *alloc(size_t n) { return nfft_malloc(n); }
+  static void release(void *p) { nfft_free(p); }
+  static bool hooked() { return nfft_malloc_hook != nullptr; }
+  static constexpr int default_m = 8;
+};
+
+// K of PRE_LIN_PSI for the Kaiser-Bessel window (reference kernel/util/window.c:37-41)
+static NFFT_INT m2K(NFFT_INT m) {
+  static const int tab[] = {1, 3, 7, 9, 14, 17, 20, 23, 24};
+  int j = (int)std::min<NFFT_INT>(m, (NFFT_INT)(sizeof(tab) / sizeof(tab[0]) - 1));
+  return (NFFT_INT)(1U << tab[j]) * (m + 2);
+}
+
+// Plan-owned x / f / f_hat (MALLOC_*) are page-locked host memory so that the
+// per-call copies run at full PCIe rate; a user malloc hook takes precedence.
+template <typename R>
+void *host_array_alloc(size_t bytes) {
+  if (Fam<R>::hooked()) return Fam<R>::alloc(bytes);
+  void *p = nullptr;
+  if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess || !p)
+    family<R>::fatal("nfft: out of pinned host memory (%zu bytes)", bytes);
+  return p;
+}
+
+template <typename R>
+void host_array_free(void *p) {
+  if (!p) return;
+  if (Fam<R>::hooked()) {
+    Fam<R>::release(p);
+    return;
+  }
+  hipHostFree(p);
+}
+
+// mv_trafo / mv_adjoint of a plan point at the family's own entry points
+// (reference kernel/nfft/nfft.c:6044-6045)
+static void set_mv(nfftf_plan *p) {
+  p->mv_trafo = reinterpret_cast<void (*)(void *)>(nfftf_trafo);
+  p->mv_adjoint = reinterpret_cast<void (*)(void *)>(nfftf_adjoint);
+}
+static void set_mv(nfft_plan *p) {
+  p->mv_trafo = reinterpret_cast<void (*)(void *)>(nfft_trafo);
+  p->mv_adjoint = reinterpret_cast<void (*)(void *)>(nfft_adjoint);
+}
+
+template <typename R, typename PLAN>
+struct PlanApi {
+  static Engine<R> *eng(const PLAN *p) {
+    Engine<R> *e = reinterpret_cast<Engine<R> *>(p->my_fftw_plan1);
+    if (!e) family<R>::fatal("nfft: plan used before nfft_init* or after nfft_finalize");
+    return e;
+  }
+
+  // reference init_help (kernel/nfft/nfft.c:5950-6046)
+  static void init_help(PLAN *p) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+      family<R>::fatal("nfft: no HIP device available (this library runs on the GPU only)");
+    if (p->flags & NFFT_OMP_BLOCKWISE_ADJOINT) p->flags |= NFFT_SORT_NODES;
+    if (p->d < 1 || p->d > kMaxDim)
+      family<R>::fatal("nfft: dimension d = %td is not supported (1 <= d <= 3)", p->d);
+    p->N_total = 1;
+    p->n_total = 1;
+    for (NFFT_INT t = 0; t < p->d; t++) {
+      p->N_total *= p->N[t];
+      p->n_total *= p->n[t];
+    }
+    p->sigma = (R *)Fam<R>::alloc(sizeof(R) * (size_t)p->d);
+    p->b = (R *)Fam<R>::alloc(sizeof(R) * (size_t)p->d);
+    for (NFFT_INT t = 0; t < p->d; t++) {
+      p->sigma[t] = (R)p->n[t] / (R)p->N[t];
+      p->b[t] = (R)(kPiL * (2.0L - 1.0L / ((long double)p->n[t] / (long double)p->N[t])));
+    }
+    if (p->flags & MALLOC_X) p->x = (R *)host_array_alloc<R>(sizeof(R) * (size_t)(p->d * p->M_total));
+    if (p->flags & MALLOC_F_HAT)
+      p->f_hat = (decltype(p->f_hat))host_array_alloc<R>(2 * sizeof(R) * (size_t)p->N_total);
+    if (p->flags & MALLOC_F)
+      p->f = (decltype(p->f))host_array_alloc<R>(2 * sizeof(R) * (size_t)p->M_total);
+    if (p->flags & PRE_PHI_HUT) {
+      p->c_phi_inv = (R **)Fam<R>::alloc(sizeof(R *) * (size_t)p->d);
+      for (NFFT_INT t = 0; t < p->d; t++) {
+        p->c_phi_inv[t] = (R *)Fam<R>::alloc(sizeof(R) * (size_t)p->N[t]);
+        long double b = kPiL * (2.0L - 1.0L / ((long double)p->n[t] / (long double)p->N[t]));
+        for (NFFT_INT ks = 0; ks < p->N[t]; ks++)
+          p->c_phi_inv[t][ks] = (R)(1.0L / phi_hut(p->n[t], ks - p->N[t] / 2, (int)p->m, b));
+      }
+    } else {
+      p->c_phi_inv = nullptr;
+    }
+    if ((p->flags & PRE_LIN_PSI) && p->K == 0) p->K = m2K(p->m);
+    p->psi = nullptr;
+    p->psi_index_g = nullptr;
+    p->psi_index_f = nullptr;
+    p->g = p->g_hat = p->g1 = p->g2 = nullptr;
+    p->spline_coeffs = nullptr;
+    p->index_x = nullptr;
+    for (int i = 0; i < 3; i++) p->MEASURE_TIME_t[i] = 0;
+    p->my_fftw_plan2 = nullptr;
+
+    long N[kMaxDim], n[kMaxDim];
+    for (NFFT_INT t = 0; t < p->d; t++) {
+      N[t] = (long)p->N[t];
+      n[t] = (long)p->n[t];
+    }
+    Engine<R> *e = new Engine<R>();
+    e->create((int)p->d, N, n, (long)p->M_total, (int)p->m);
+    p->my_fftw_plan1 = reinterpret_cast<decltype(p->my_fftw_plan1)>(e);
+    set_mv(p);
+  }
+
+  static void init_guru(PLAN *p, int d, int *N, int M, int *n, int m, unsigned flags,
+                        unsigned fftw_flags) {
+    p->d = d;
+    p->M_total = M;
+    p->N = (NFFT_INT *)Fam<R>::alloc(sizeof(NFFT_INT) * (size_t)d);
+    p->n = (NFFT_INT *)Fam<R>::alloc(sizeof(NFFT_INT) * (size_t)d);
+    for (int t = 0; t < d; t++) {
+      p->N[t] = N[t];
+      p->n[t] = n[t];
+    }
+    p->m = m;
+    p->flags = flags;
+    p->fftw_flags = fftw_flags;
+    p->K = 0;
+    init_help(p);
+  }
+
+  static void init_lin(PLAN *p, int d, int *N, int M, int *n, int m, int K, unsigned flags,
+                       unsigned fftw_flags) {
+    p->d = d;
+    p->M_total = M;
+    p->N = (NFFT_INT *)Fam<R>::alloc(sizeof(NFFT_INT) * (size_t)d);
+    p->n = (NFFT_INT *)Fam<R>::alloc(sizeof(NFFT_INT) * (size_t)d);
+    for (int t = 0; t < d; t++) {
+      p->N[t] = N[t];
+      p->n[t] = n[t];
+    }
+    p->m = m;
+    p->flags = flags;
+    p->fftw_flags = fftw_flags;
+    p->K = K;
+    init_help(p);
+  }
+
+  // defaults of the reference X(init) (kernel/nfft/nfft.c:6048-6087)
+  static void init(PLAN *p, int d, int *N, int M) {
+    p->d = d;
+    p->N = (NFFT_INT *)Fam<R>::alloc(sizeof(NFFT_INT) * (size_t)d);
+    p->n = (NFFT_INT *)Fam<R>::alloc(sizeof(NFFT_INT) * (size_t)d);
+    for (int t = 0; t < d; t++) {
+      p->N[t] = N[t];
+      NFFT_INT np2 = 1;
+      if (N[t] < 2) np2 = N[t] + 1;
+      else
+        while (np2 < N[t]) np2 <<= 1;
+      p->n[t] = 2 * np2;
+    }
+    p->M_total = M;
+    p->m = Fam<R>::default_m;
+    if (d > 1)
+      p->flags = PRE_PHI_HUT | PRE_PSI | MALLOC_X | MALLOC_F_HAT | MALLOC_F | FFTW_INIT |
+                 NFFT_SORT_NODES;
+    else
+      p->flags = PRE_PHI_HUT | PRE_PSI | MALLOC_X | MALLOC_F_HAT | MALLOC_F | FFTW_INIT |
+                 FFT_OUT_OF_PLACE;
+    p->fftw_flags = FFTW_ESTIMATE | FFTW_DESTROY_INPUT;
+    p->K = 0;
+    init_help(p);
+  }
+
+  static void precompute(PLAN *p) {
+    Engine<R> *e = eng(p);
+    e->upload_nodes(p->x);
+    e->build_nodes();
+  }
+
+  static void ensure_nodes(PLAN *p, Engine<R> *e) {
+    if (!(p->flags & PRE_ONE_PSI) || !e->nodes_ready) precompute(p);
+  }
+
+  static void store_times(PLAN *p, Engine<R> *e, bool is_adjoint) {
+    if (!e->timing) return;
+    double t[3];
+    e->stage_times(t);
+    if (is_adjoint) {  // recorded order B, FFT, D
+      p->MEASURE_TIME_t[0] = (R)t[2];
+      p->MEASURE_TIME_t[1] = (R)t[1];
+      p->MEASURE_TIME_t[2] = (R)t[0];
+    } else {
+      p->MEASURE_TIME_t[0] = (R)t[0];
+      p->MEASURE_TIME_t[1] = (R)t[1];
+      p->MEASURE_TIME_t[2] = (R)t[2];
+    }
+  }
+
+  static void trafo_direct(const PLAN *p) {
+    Engine<R> *e = eng(p);
+    e->upload_nodes(p->x);
+    if (p->N_total > 0)
+      NFFT_HIP_CHECK(R, hipMemcpyAsync(e->fhat, p->f_hat, sizeof(cplx<R>) * (size_t)p->N_total,
+                                       hipMemcpyHostToDevice, e->stream));
+    e->trafo_direct(e->fhat, e->f);
+    if (p->M_total > 0)
+      NFFT_HIP_CHECK(R, hipMemcpyAsync(p->f, e->f, sizeof(cplx<R>) * (size_t)p->M_total,
+                                       hipMemcpyDeviceToHost, e->stream));
+    NFFT_HIP_CHECK(R, hipStreamSynchronize(e->stream));
+  }
+
+  static void adjoint_direct(const PLAN *p) {
+    Engine<R> *e = eng(p);
+    e->upload_nodes(p->x);
+    if (p->M_total > 0)
+      NFFT_HIP_CHECK(R, hipMemcpyAsync(e->f, p->f, sizeof(cplx<R>) * (size_t)p->M_total,
+                                       hipMemcpyHostToDevice, e->stream));
+    e->adjoint_direct(e->f, e->fhat);
+    if (p->N_total > 0)
+      NFFT_HIP_CHECK(R, hipMemcpyAsync(p->f_hat, e->fhat, sizeof(cplx<R>) * (size_t)p->N_total,
+                                       hipMemcpyDeviceToHost, e->stream));
+    NFFT_HIP_CHECK(R, hipStreamSynchronize(e->stream));
+  }
+
+  static void trafo(PLAN *p) {
+    Engine<R> *e = eng(p);
+    if (e->direct_only) {
+      trafo_direct(p);
+      return;
+    }
+    ensure_nodes(p, e);
+    NFFT_HIP_CHECK(R, hipMemcpyAsync(e->fhat, p->f_hat, sizeof(cplx<R>) * (size_t)p->N_total,
+                                     hipMemcpyHostToDevice, e->stream));
+    e->trafo(e->fhat, e->f);
+    if (p->M_total > 0)
+      NFFT_HIP_CHECK(R, hipMemcpyAsync(p->f, e->f, sizeof(cplx<R>) * (size_t)p->M_total,
+                                       hipMemcpyDeviceToHost, e->stream));
+    NFFT_HIP_CHECK(R, hipStreamSynchronize(e->stream));
+    store_times(p, e, false);
+  }
+
+  static void adjoint(PLAN *p) {
+    Engine<R> *e = eng(p);
+    if (e->direct_only) {
+      adjoint_direct(p);
+      return;
+    }
+    ensure_nodes(p, e);
+    if (p->M_total > 0)
+      NFFT_HIP_CHECK(R, hipMemcpyAsync(e->f, p->f, sizeof(cplx<R>) * (size_t)p->M_total,
+                                       hipMemcpyHostToDevice, e->stream));
+    e->adjoint(e->f, e->fhat);
+    NFFT_HIP_CHECK(R, hipMemcpyAsync(p->f_hat, e->fhat, sizeof(cplx<R>) * (size_t)p->N_total,
+                                     hipMemcpyDeviceToHost, e->stream));
+    NFFT_HIP_CHECK(R, hipStreamSynchronize(e->stream));
+    store_times(p, e, true);
+  }
+
+  static void trafo_device(PLAN *p, const void *fh, void *f) {
+    Engine<R> *e = eng(p);
+    if (e->direct_only) {
+      if (!e->nodes_ready) precompute(p);
+      e->trafo_direct((const cplx<R> *)fh, (cplx<R> *)f);
+      return;
+    }
+    ensure_nodes(p, e);
+    e->trafo((const cplx<R> *)fh, (cplx<R> *)f);
+    if (e->timing) store_times(p, e, false);
+  }
+
+  static void adjoint_device(PLAN *p, const void *f, void *fh) {
+    Engine<R> *e = eng(p);
+    if (e->direct_only) {
+      if (!e->nodes_ready) precompute(p);
+      e->adjoint_direct((const cplx<R> *)f, (cplx<R> *)fh);
+      return;
+    }
+    ensure_nodes(p, e);
+    e->adjoint((const cplx<R> *)f, (cplx<R> *)fh);
+    if (e->timing) store_times(p, e, true);
+  }
+
+  // reference X(check) (kernel/nfft/nfft.c:6169-6207)
+  static const char *check(PLAN *p) {
+    if (!p->f) return "Member f not initialized.";
+    if (!p->x) return "Member x not initialized.";
+    if (!p->f_hat) return "Member f_hat not initialized.";
+    if ((p->flags & PRE_LIN_PSI) && p->K < p->M_total)
+      return "Number of nodes too small to use PRE_LIN_PSI.";
+    for (NFFT_INT j = 0; j < p->M_total * p->d; j++)
+      if (p->x[j] < R(-0.5) || p->x[j] >= R(0.5)) return "ths->x out of range [-0.5,0.5)";
+    for (NFFT_INT t = 0; t < p->d; t++) {
+      if (p->sigma[t] <= 1) return "Oversampling factor too small";
+      if (p->N[t] % 2 == 1) return "polynomial degree N has to be even";
+    }
+    return nullptr;
+  }
+
+  // reference X(finalize) (kernel/nfft/nfft.c:6209-6270)
+  static void finalize(PLAN *p) {
+    Engine<R> *e = reinterpret_cast<Engine<R> *>(p->my_fftw_plan1);
+    if (e) {
+      e->destroy();
+      delete e;
+    }
+    p->my_fftw_plan1 = nullptr;
+    if ((p->flags & PRE_PHI_HUT) && p->c_phi_inv) {
+      for (NFFT_INT t = 0; t < p->d; t++) Fam<R>::release(p->c_phi_inv[t]);
+      Fam<R>::release(p->c_phi_inv);
+      p->c_phi_inv = nullptr;
+    }
+    if (p->flags & MALLOC_F) host_array_free<R>(p->f);
+    if (p->flags & MALLOC_F_HAT) host_array_free<R>(p->f_hat);
+    if (p->flags & MALLOC_X) host_array_free<R>(p->x);
+    Fam<R>::release(p->b);
+    Fam<R>::release(p->sigma);
+    Fam<R>::release(p->n);
+    Fam<R>::release(p->N);
+  }
+
+  static void set_stream(PLAN *p, void *s) {
+    Engine<R> *e = eng(p);
+    e->stream = s ? (hipStream_t)s : e->own_stream;
+  }
+  static void *get_stream(PLAN *p) { return (void *)eng(p)->stream; }
+  static void synchronize(PLAN *p) { NFFT_HIP_CHECK(R, hipStreamSynchronize(eng(p)->stream)); }
+  static void set_timing(PLAN *p, int on) { eng(p)->timing = on != 0; }
+  static long num_subproblems(PLAN *p) { return eng(p)->nsub; }
+};
+
+}  // namespace nfft_hip
+
+using nfft_hip::PlanApi;
+
+#define NFFT_HIP_EXPORT_PLAN(P, R, PLAN)                                                       \
+  extern "C" {                                                                                \
+  void P##init_guru(PLAN *p, int d, int *N, int M, int *n, int m, unsigned fl, unsigned ff) { \
+    PlanApi<R, PLAN>::init_guru(p, d, N, M, n, m, fl, ff);                                    \
+  }                                                                                           \
+  void P##init_lin(PLAN *p, int d, int *N, int M, int *n, int m, int K, unsigned fl,          \
+                   unsigned ff) {                                                             \
+    PlanApi<R, PLAN>::init_lin(p, d, N, M, n, m, K, fl, ff);                                  \
+  }                                                                                           \
+  void P##init(PLAN *p, int d, int *N, int M) { PlanApi<R, PLAN>::init(p, d, N, M); }         \
+  void P##init_1d(PLAN *p, int N1, int M) {                                                   \
+    int N[1] = {N1};                                                                          \
+    PlanApi<R, PLAN>::init(p, 1, N, M);                                                       \
+  }                                                                                           \
+  void P##init_2d(PLAN *p, int N1, int N2, int M) {                                           \
+    int N[2] = {N1, N2};                                                                      \
+    PlanApi<R, PLAN>::init(p, 2, N, M);                                                       \
+  }                                                                                           \
+  void P##init_3d(PLAN *p, int N1, int N2, int N3, int M) {                                   \
+    int N[3] = {N1, N2, N3};                                                                  \
+    PlanApi<R, PLAN>::init(p, 3, N, M);                                                       \
+  }                                                                                           \
+  void P##precompute_one_psi(PLAN *p) { PlanApi<R, PLAN>::precompute(p); }                    \
+  void P##precompute_psi(PLAN *p) { PlanApi<R, PLAN>::precompute(p); }                        \
+  void P##precompute_full_psi(PLAN *p) { PlanApi<R, PLAN>::precompute(p); }                   \
+  void P##precompute_fg_psi(PLAN *p) { PlanApi<R, PLAN>::precompute(p); }                     \
+  void P##precompute_lin_psi(PLAN *p) { PlanApi<R, PLAN>::precompute(p); }                    \
+  void P##trafo(PLAN *p) { PlanApi<R, PLAN>::trafo(p); }                                      \
+  void P##trafo_1d(PLAN *p) { PlanApi<R, PLAN>::trafo(p); }                                   \
+  void P##trafo_2d(PLAN *p) { PlanApi<R, PLAN>::trafo(p); }                                   \
+  void P##trafo_3d(PLAN *p) { PlanApi<R, PLAN>::trafo(p); }                                   \
+  void P##adjoint(PLAN *p) { PlanApi<R, PLAN>::adjoint(p); }                                  \
+  void P##adjoint_1d(PLAN *p) { PlanApi<R, PLAN>::adjoint(p); }                               \
+  void P##adjoint_2d(PLAN *p) { PlanApi<R, PLAN>::adjoint(p); }                               \
+  void P##adjoint_3d(PLAN *p) { PlanApi<R, PLAN>::adjoint(p); }                               \
+  void P##trafo_direct(const PLAN *p) { PlanApi<R, PLAN>::trafo_direct(p); }                  \
+  void P##adjoint_direct(const PLAN *p) { PlanApi<R, PLAN>::adjoint_direct(p); }              \
+  const char *P##check(PLAN *p) { return PlanApi<R, PLAN>::check(p); }                        \
+  void P##finalize(PLAN *p) { PlanApi<R, PLAN>::finalize(p); }                                \
+  void P##trafo_device(PLAN *p, const void *fh, void *f) {                                    \
+    PlanApi<R, PLAN>::trafo_device(p, fh, f);                                                 \
+  }                                                                                           \
+  void P##adjoint_device(PLAN *p, const void *f, void *fh) {                                  \
+    PlanApi<R, PLAN>::adjoint_device(p, f, fh);                                               \
+  }                                                                                           \
+  void P##set_stream(PLAN *p, void *s) { PlanApi<R, PLAN>::set_stream(p, s); }                \
+  void *P##get_stream(PLAN *p) { return PlanApi<R, PLAN>::get_stream(p); }                    \
+  void P##synchronize(PLAN *p) { PlanApi<R, PLAN>::synchronize(p); }                          \
+  void P##set_timing(PLAN *p, int on) { PlanApi<R, PLAN>::set_timing(p, on); }                \
+  void P##timing_begin(PLAN *p) { PlanApi<R, PLAN>::eng(p)->timing_begin(); }                 \
+  void P##timing_end(PLAN *p, double *out) { PlanApi<R, PLAN>::eng(p)->timing_end(out); }     \
+  long P##num_subproblems(PLAN *p) { return PlanApi<R, PLAN>::num_subproblems(p); }           \
+  }
+
+NFFT_HIP_EXPORT_PLAN(nfftf_, float, nfftf_plan)
+NFFT_HIP_EXPORT_PLAN(nfft_, double, nfft_plan)
+
+namespace nfft_hip {
+
+template void *host_array_alloc<float>(size_t);
+template void *host_array_alloc<double>(size_t);
+template void host_array_free<float>(void *);
+template void host_array_free<double>(void *);
+
+// The solver reaches the plan only through the mv prefix; it is one of this
+// library's plans exactly when its mv_trafo is this library's trafo.
+template <>
+Engine<float> *engine_of_mv<float>(void *mv) {
+  nfftf_plan *p = static_cast<nfftf_plan *>(mv);
+  if (p->mv_trafo != reinterpret_cast<void (*)(void *)>(nfftf_trafo)) return nullptr;
+  return reinterpret_cast<Engine<float> *>(p->my_fftw_plan1);
+}
+
+template <>
+Engine<double> *engine_of_mv<double>(void *mv) {
+  nfft_plan *p = static_cast<nfft_plan *>(mv);
+  if (p->mv_trafo != reinterpret_cast<void (*)(void *)>(nfft_trafo)) return nullptr;
+  return reinterpret_cast<Engine<double> *>(p->my_fftw_plan1);
+}
+
+}  // namespace nfft_hip
+
+namespace nfft_hip {
+
+template <>
+void ensure_nodes_of_mv<float>(void *mv) {
+  nfftf_plan *p = static_cast<nfftf_plan *>(mv);
+  PlanApi<float, nfftf_plan>::ensure_nodes(p, PlanApi<float, nfftf_plan>::eng(p));
+}
+
+template <>
+void ensure_nodes_of_mv<double>(void *mv) {
+  nfft_plan *p = static_cast<nfft_plan *>(mv);
+  PlanApi<double, nfft_plan>::ensure_nodes(p, PlanApi<double, nfft_plan>::eng(p));
+}
+
+}  // namespace nfft_hip

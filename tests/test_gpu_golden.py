@@ -1,0 +1,91 @@
+"""The reference's own unit tests, run against the GPU library through the C ABI.
+
+Mirrors tests/nfft.c of the reference: every golden file (Mathematica NDFT at
+64 digits) crossed with the plan initialisers the reference uses
+(init_1d/2d/3d, init, init_guru with PRE_PSI, init_guru with PRE_FULL_PSI)
+and the entry points (trafo, trafo_Nd, adjoint, adjoint_Nd, trafo_direct,
+adjoint_direct), for both precision families.  Error measures
+(tests/nfft.c:739-781): trafo ||f - f_ref||_inf / ||f_hat||_1, adjoint
+||f_hat - f_hat_ref||_inf / ||f||_1.  Bounds (tests/nfft.c:211-284): direct
+48 eps; fast max(a err_KB(m), b eps, 48 eps).  Cases that nfft_check rejects
+count as passed, as in the reference (tests/nfft.c:313-324).
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import nfft_amd as nf
+import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "nfft")
+FILES = sorted(os.path.basename(f) for f in glob.glob(os.path.join(GOLD, "nfft_*.txt")))
+
+DEFAULT = nf.MALLOC_X | nf.MALLOC_F | nf.MALLOC_F_HAT | nf.FFTW_INIT | nf.FFT_OUT_OF_PLACE
+INITS = {
+    "init_Nd": dict(how="init_Nd"),
+    "init": dict(how="init"),
+    "guru_pre_psi": dict(flags=nf.PRE_PHI_HUT | nf.PRE_PSI | DEFAULT),
+    "guru_pre_full_psi": dict(flags=nf.PRE_PHI_HUT | nf.PRE_FULL_PSI | DEFAULT),
+}
+
+
+def _run(name, precision, init, entry):
+    g = O.read_golden(os.path.join(GOLD, name))
+    N, M = g["N"], g["M"]
+    adjoint = "adjoint" in name
+    kw = dict(INITS[init])
+    if "flags" in kw:
+        kw["n"] = [2 * O.next_power_of_2(t) for t in N]
+        kw["m"] = 8 if precision == "double" else 4
+    p = nf.Plan(N, M, precision=precision, **kw)
+    try:
+        p.x[:] = g["x"]
+        direct = entry.endswith("direct")
+        if not direct:
+            msg = p.check()
+            if msg is not None:
+                return None
+        if p.p.flags & nf.PRE_ONE_PSI:
+            p.precompute()
+        if adjoint:
+            p.f[:] = g["f"]
+        else:
+            p.f_hat[:] = g["f_hat"]
+        if entry == "Nd":
+            p.call(("adjoint_%dd" if adjoint else "trafo_%dd") % len(N))
+        elif entry == "fast":
+            p.adjoint() if adjoint else p.trafo()
+        else:
+            p.adjoint_direct() if adjoint else p.trafo_direct()
+        if adjoint:
+            err = O.err_adjoint_measure(g["f_hat"], p.f_hat.astype(np.complex128), p.f.astype(np.complex128))
+        else:
+            err = O.err_trafo_measure(g["f"], p.f.astype(np.complex128), p.f_hat.astype(np.complex128))
+        bound = O.err_bound_direct(precision) if direct else O.err_bound_fast(p.m, precision)
+        return err, bound
+    finally:
+        p.finalize()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("name", FILES)
+def test_golden_fast(name, precision):
+    for init in INITS:
+        for entry in ("fast", "Nd"):
+            r = _run(name, precision, init, entry)
+            if r is None:
+                continue
+            err, bound = r
+            assert err < bound, (name, precision, init, entry, err, bound)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("name", FILES)
+def test_golden_direct(name, precision):
+    r = _run(name, precision, "guru_pre_psi", "direct")
+    err, bound = r
+    assert err < bound, (name, precision, err, bound)

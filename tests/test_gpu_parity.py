@@ -1,0 +1,197 @@
+"""GPU library vs the FP64 CPU oracle at workload-shaped sizes, through the C ABI.
+
+Every case states relative l2 and l-infinity errors against the oracle run on
+the same inputs (for the float family, the inputs rounded to float first, so
+only arithmetic error is measured).  Tolerances:
+  double family: rel l2 <= 1e-12 vs the oracle's identical algorithm
+  float family:  rel l2 <= 1e-5 (target of the FP32 path) and the reference's
+                 own fast-transform bound under its error measure
+Full-size cases (the 3-D kooshball at M = 2^24) are checked through
+size-independent properties: adjointness, linearity, and float vs double
+agreement.
+"""
+import numpy as np
+import pytest
+
+import nfft_amd as nf
+import oracle as O
+from nfft_amd import workloads as W
+
+FLAGS = nf.PRE_PHI_HUT | nf.PRE_PSI | nf.MALLOC_X | nf.MALLOC_F_HAT | nf.MALLOC_F | nf.FFTW_INIT | nf.FFT_OUT_OF_PLACE
+
+
+def rel_errs(a, ref):
+    a = np.asarray(a, dtype=np.complex128)
+    ref = np.asarray(ref, dtype=np.complex128)
+    l2 = np.linalg.norm(a - ref) / np.linalg.norm(ref)
+    linf = np.max(np.abs(a - ref)) / np.max(np.abs(ref))
+    return l2, linf
+
+
+def make_plan(N, x, m, precision, n=None):
+    n = n or [2 * O.next_power_of_2(t) for t in N]
+    p = nf.Plan(N, x.shape[0], n=n, m=m, flags=FLAGS, precision=precision)
+    p.x[:] = x
+    p.precompute()
+    return p
+
+
+def run_pair(N, x, fh, f, m, precision, n=None):
+    """GPU trafo and adjoint; oracle on the same (rounded) inputs."""
+    p = make_plan(N, x, m, precision, n)
+    try:
+        xr = p.x.astype(np.float64).copy()
+        p.f_hat[:] = fh
+        fhr = p.f_hat.astype(np.complex128).copy()
+        p.trafo()
+        f_gpu = p.f.astype(np.complex128).copy()
+        p.f[:] = f
+        fr = p.f.astype(np.complex128).copy()
+        p.adjoint()
+        fh_gpu = p.f_hat.astype(np.complex128).copy()
+        nn = p.n
+    finally:
+        p.finalize()
+    f_ref = O.nfft_trafo(N, nn, m, xr, fhr)
+    fh_ref = O.nfft_adjoint(N, nn, m, xr, fr)
+    return (f_gpu, f_ref, fhr), (fh_gpu, fh_ref, fr)
+
+
+CASES = [
+    # name, N, node generator, m
+    ("1d_cfg1", [1024], lambda: W.uniform(4096, 1, seed=11), 6),
+    ("1d_small", [64], lambda: W.uniform(300, 1, seed=12), 4),
+    ("2d_clustered_m4", [64, 64], lambda: W.clustered_2d(8192, seed=13), 4),
+    ("2d_clustered_m6", [64, 48], lambda: W.clustered_2d(6000, seed=14), 6),
+    ("2d_radial", [64, 64], lambda: W.golden_radial(40, 128), 4),
+    ("3d_kooshball_m4", [16, 16, 16], lambda: W.kooshball(64, 64, seed=15), 4),
+    ("3d_uniform_m3", [12, 16, 20], lambda: W.uniform(2000, 3, seed=16), 3),
+    ("3d_m2", [8, 8, 8], lambda: W.uniform(500, 3, seed=17), 2),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_vs_oracle(case, precision):
+    name, N, gen, m = case
+    x = gen()
+    NN = int(np.prod(N))
+    fh = W.complex_normal(NN, seed=21)
+    f = W.complex_normal(x.shape[0], seed=22)
+    (fg, fr, fhr), (hg, hr, frr) = run_pair(N, x, fh, f, m, precision)
+    l2t, lit = rel_errs(fg, fr)
+    l2a, lia = rel_errs(hg, hr)
+    print("%s %s trafo l2 %.2e linf %.2e | adjoint l2 %.2e linf %.2e" % (name, precision, l2t, lit, l2a, lia))
+    if precision == "double":
+        assert l2t < 1e-12 and l2a < 1e-12
+    else:
+        assert l2t < 1e-5 and l2a < 1e-5
+        bound = O.err_bound_fast(m, "single")
+        assert O.err_trafo_measure(fr, fg, fhr) < bound
+        assert O.err_adjoint_measure(hr, hg, frr) < bound
+
+
+@pytest.mark.gpu
+def test_cfg1_vs_ndft():
+    """1-D N=1024, M=4096, sigma=2, m=6, FP64 against the direct NDFT under the
+    reference's bound (tests/nfft.c:217-284)."""
+    N = [1024]
+    x = W.uniform(4096, 1, seed=11)
+    fh = W.complex_normal(1024, seed=21)
+    f = W.complex_normal(4096, seed=22)
+    (fg, _, fhr), (hg, _, frr) = run_pair(N, x, fh, f, 6, "double")
+    f_nd = O.ndft_trafo(N, x, fh)
+    fh_nd = O.ndft_adjoint(N, x, f)
+    bound = O.err_bound_fast(6, "double")
+    assert O.err_trafo_measure(f_nd, fg, fh) < bound
+    assert O.err_adjoint_measure(fh_nd, hg, f) < bound
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_adjointness_and_direct(precision):
+    N = [32, 24]
+    x = W.clustered_2d(3000, seed=31)
+    p = make_plan(N, x, 4, precision)
+    try:
+        fh = W.complex_normal(p.p.N_total, seed=32)
+        f = W.complex_normal(p.M, seed=33)
+        p.f_hat[:] = fh
+        p.trafo()
+        Af = p.f.astype(np.complex128).copy()
+        p.trafo_direct()
+        Af_direct = p.f.astype(np.complex128).copy()
+        p.f[:] = f
+        p.adjoint()
+        AHf = p.f_hat.astype(np.complex128).copy()
+        fhr = np.asarray(fh, dtype=p.cdtype).astype(np.complex128)
+        frr = np.asarray(f, dtype=p.cdtype).astype(np.complex128)
+        lhs = np.vdot(frr, Af)
+        rhs = np.vdot(AHf, fhr)
+        tol = 1e-12 if precision == "double" else 2e-6
+        assert abs(lhs - rhs) <= tol * abs(lhs)
+        # fast vs direct under the reference bound
+        assert O.err_trafo_measure(Af_direct, Af, fhr) < O.err_bound_fast(4, precision)
+    finally:
+        p.finalize()
+
+
+@pytest.mark.gpu
+def test_empty_and_edge_sizes():
+    # M = 0: trafo is a no-op, adjoint yields zeros
+    p = nf.Plan([16, 16], 0, n=[32, 32], m=4, flags=FLAGS, precision="single")
+    try:
+        p.precompute()
+        p.f_hat[:] = 1.0
+        p.trafo()
+        p.adjoint()
+        assert np.all(p.f_hat == 0)
+    finally:
+        p.finalize()
+    # all nodes in one cell (worst bin imbalance) and nodes at the -0.5 edge
+    x = np.zeros((5000, 2))
+    x[::2] = -0.5
+    fh = W.complex_normal(16 * 16, seed=41)
+    f = W.complex_normal(5000, seed=42)
+    (fg, fr, _), (hg, hr, _) = run_pair([16, 16], x, fh, f, 4, "double")
+    assert rel_errs(fg, fr)[0] < 1e-12 and rel_errs(hg, hr)[0] < 1e-12
+    # direct fallback when N <= m
+    (fg, fr, _), (hg, hr, _) = run_pair([4, 6], W.uniform(50, 2, seed=43), W.complex_normal(24, seed=44),
+                                        W.complex_normal(50, seed=45), 6, "double")
+    assert rel_errs(fg, fr)[0] < 1e-12 and rel_errs(hg, hr)[0] < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_non_power_of_two_grid(precision):
+    # oversampled lengths with factors 3, 5 exercise the mixed-radix FFT
+    N = [20, 12]
+    x = W.uniform(700, 2, seed=51)
+    fh = W.complex_normal(240, seed=52)
+    f = W.complex_normal(700, seed=53)
+    (fg, fr, _), (hg, hr, _) = run_pair(N, x, fh, f, 4, precision, n=[60, 30])
+    tol = 1e-12 if precision == "double" else 1e-5
+    assert rel_errs(fg, fr)[0] < tol and rel_errs(hg, hr)[0] < tol
+
+
+@pytest.mark.gpu
+def test_pointer_swap_between_calls():
+    """The solver swaps f / f_hat pointers between calls; host pointers are
+    authoritative on every call."""
+    N = [32, 32]
+    x = W.uniform(1000, 2, seed=61)
+    p = make_plan(N, x, 4, "double")
+    try:
+        a = np.ascontiguousarray(W.complex_normal(1024, seed=62))
+        out = np.zeros(1000, dtype=np.complex128)
+        import ctypes
+        saved_fh, saved_f = p.p.f_hat, p.p.f
+        p.p.f_hat = ctypes.cast(a.ctypes.data, type(p.p.f_hat))
+        p.p.f = ctypes.cast(out.ctypes.data, type(p.p.f))
+        p.trafo()
+        p.p.f_hat, p.p.f = saved_fh, saved_f
+        ref = O.nfft_trafo(N, p.n, 4, x, a)
+        assert rel_errs(out, ref)[0] < 1e-12
+    finally:
+        p.finalize()
