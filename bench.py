@@ -160,7 +160,7 @@ def main():
     b_trafo_ms = stages_ms["trafo_B"]
     b_adj_ms = stages_ms["adjoint_B"]
     if b_adj_ms >= b_trafo_ms:
-        dom_name, dom_ms = "B^T spreading (k_spread_tile, adjoint)", b_adj_ms
+        dom_name, dom_ms = "B^T spreading (k_bwave_tile, adjoint)", b_adj_ms
     else:
         dom_name, dom_ms = "B interpolation (k_interp_tile, trafo)", b_trafo_ms
     achieved = ab["B"] / (dom_ms * 1e-3) / 1e9

@@ -40,9 +40,10 @@ extern "C"
   /* Deferred stage timing for throughput runs: between timing_begin and     \
    * timing_end every trafo / adjoint records HIP events around its stages   \
    * on the plan's stream without waiting.  timing_end waits and writes     \
-   * out[0..7] = number of trafo calls, number of adjoint calls, then the    \
+   * out[0..8] = number of trafo calls, number of adjoint calls, then the    \
    * summed seconds of trafo D, trafo FFT, trafo B (one interpolation        \
-   * kernel), adjoint B (one spreading kernel), adjoint FFT, adjoint D. */   \
+   * kernel), adjoint B^T spreading (one kernel), adjoint tile reduction     \
+   * (one kernel, 0 on the atomic path), adjoint FFT, adjoint D^T. */        \
   void P##timing_begin(PLAN *ths);                                           \
   void P##timing_end(PLAN *ths, double *out);                                \
   /* Number of B-stage work items (sub-problems) built by the last           \

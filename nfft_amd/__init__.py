@@ -281,10 +281,10 @@ class Plan:
     def timing_end(self):
         """Summed stage seconds since timing_begin: dict with call counts and
         trafo D / FFT / B, adjoint B / FFT / D."""
-        out = (ctypes.c_double * 8)()
+        out = (ctypes.c_double * 9)()
         self._fn("timing_end")(self.pp, out)
         keys = ["n_trafo", "n_adjoint", "trafo_D", "trafo_FFT", "trafo_B",
-                "adjoint_B", "adjoint_FFT", "adjoint_D"]
+                "adjoint_B", "adjoint_Bred", "adjoint_FFT", "adjoint_D"]
         return dict(zip(keys, list(out)))
 
     def num_subproblems(self):

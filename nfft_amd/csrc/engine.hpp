@@ -63,9 +63,9 @@ template <typename R> struct Engine {
   hipStream_t stream = nullptr;
   hipStream_t own_stream = nullptr;
   bool timing = false;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool deferred = false;  // timing_begin .. timing_end
-  std::vector<std::array<hipEvent_t, 4>> log_ev;
+  std::vector<std::array<hipEvent_t, 5>> log_ev;
   std::vector<int> log_kind;  // 0 trafo, 1 adjoint
   size_t log_n = 0;
 
@@ -78,9 +78,19 @@ template <typename R> struct Engine {
   int *cell = nullptr;
   R *frac = nullptr;
   int *perm = nullptr;
-  int4 *subs = nullptr;
+  int4 *subs = nullptr;  // interpolation work items
   long nsub = 0;
   long subs_cap = 0;
+  // spreading work items; per bin its first item and item count
+  int4 *sp_subs = nullptr;
+  long sp_nsub = 0, sp_subs_cap = 0;
+  int *sp_first = nullptr, *sp_count = nullptr;
+  long bin_arrays_cap = 0;
+  int spread_max_per_sub = 2048;
+  // spreading tiles of all work items (no HBM atomics); null: atomic flush
+  cplx<R> *tiles = nullptr;
+  long tiles_cap = 0;
+  bool tile_scratch = false;
   int *scratch_i = nullptr;  // binning scratch
   long scratch_cap = 0;
   bool nodes_ready = false;
@@ -88,6 +98,10 @@ template <typename R> struct Engine {
   BinGeom geom{};
   WinParams<R> win{};
   int max_per_sub = 1024;
+  // B kernel shapes: 0 lane per node, 1 wave per node (d = 2, 3).  Set at
+  // plan creation; NFFT_HIP_INTERP / NFFT_HIP_SPREAD = lane | wave override.
+  int interp_variant = 0;
+  int spread_variant = 1;
   std::vector<FftAxis<R>> axes;
 
   void create(int d_, const long *N_, const long *n_, long M_, int m_);
@@ -98,7 +112,7 @@ template <typename R> struct Engine {
   void adjoint(const cplx<R> *f_dev, cplx<R> *fhat_dev);
   void trafo_direct(const cplx<R> *fhat_dev, cplx<R> *f_dev);
   void adjoint_direct(const cplx<R> *f_dev, cplx<R> *fhat_dev);
-  void stage_times(double out[3]);
+  void stage_times(double out[4]);
   hipEvent_t *call_events(int kind);
   void timing_begin();
   void timing_end(double *out);
@@ -128,6 +142,8 @@ template <typename R>
 void launch_interp(const cplx<R> *g, cplx<R> *f, const Engine<R> &e, hipStream_t s);
 template <typename R>
 void launch_spread(const cplx<R> *f, cplx<R> *g, const Engine<R> &e, hipStream_t s);
+template <typename R>
+void launch_gather_tiles(const Engine<R> &e, cplx<R> *g, hipStream_t s);
 template <typename R>
 void launch_ndft(const R *x, const cplx<R> *in, cplx<R> *out, int d, const long *N, long M,
                  int adjoint, hipStream_t s);

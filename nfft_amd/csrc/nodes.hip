@@ -143,22 +143,36 @@ void bin_nodes(Engine<R> &e) {
   if (M > 0)
     hipLaunchKernelGGL(k_node_place<R>, dim3((unsigned)ceil_div(M, 256)), dim3(256), 0, s, gg,
                        cell_tmp, frac_tmp, bin_of, rank_of, bin_start, e.cell, e.frac, e.perm);
-  hipLaunchKernelGGL(k_count_subs, dim3((unsigned)ceil_div(nbins, 256)), dim3(256), 0, s, hist,
-                     nbins, e.max_per_sub, nsub_of);
-  hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, nsub_of, sub_start, nbins,
-                     sums + 1);
-  int total = 0;
-  NFFT_HIP_CHECK(R, hipMemcpyAsync(&total, sums + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-  NFFT_HIP_CHECK(R, hipStreamSynchronize(s));
-  if (total > e.subs_cap) {
-    if (e.subs) NFFT_HIP_CHECK(R, hipFree(e.subs));
-    NFFT_HIP_CHECK(R, hipMalloc(&e.subs, (size_t)total * sizeof(int4)));
-    e.subs_cap = total;
+  // work items of interpolation (small, for balance) and of spreading
+  // (larger, to bound the number of tiles written to scratch); the spreading
+  // list keeps its per-bin first item and count for the tile reduction
+  auto make_list = [&](int max_per_sub, int *first_of, int *count_of, int4 **subs, long *cap,
+                       long *nsub) {
+    hipLaunchKernelGGL(k_count_subs, dim3((unsigned)ceil_div(nbins, 256)), dim3(256), 0, s, hist,
+                       nbins, max_per_sub, count_of);
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, count_of, first_of, nbins,
+                       sums + 1);
+    int total = 0;
+    NFFT_HIP_CHECK(R, hipMemcpyAsync(&total, sums + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    NFFT_HIP_CHECK(R, hipStreamSynchronize(s));
+    if (total > *cap) {
+      if (*subs) NFFT_HIP_CHECK(R, hipFree(*subs));
+      NFFT_HIP_CHECK(R, hipMalloc(subs, (size_t)total * sizeof(int4)));
+      *cap = total;
+    }
+    *nsub = total;
+    if (total > 0)
+      hipLaunchKernelGGL(k_fill_subs, dim3((unsigned)ceil_div(nbins, 256)), dim3(256), 0, s, hist,
+                         bin_start, first_of, nbins, max_per_sub, *subs);
+  };
+  make_list(e.max_per_sub, sub_start, nsub_of, &e.subs, &e.subs_cap, &e.nsub);
+  if (nbins > e.bin_arrays_cap) {
+    if (e.sp_first) NFFT_HIP_CHECK(R, hipFree(e.sp_first));
+    NFFT_HIP_CHECK(R, hipMalloc(&e.sp_first, 2 * nbins * sizeof(int)));
+    e.bin_arrays_cap = nbins;
   }
-  e.nsub = total;
-  if (total > 0)
-    hipLaunchKernelGGL(k_fill_subs, dim3((unsigned)ceil_div(nbins, 256)), dim3(256), 0, s, hist,
-                       bin_start, sub_start, nbins, e.max_per_sub, e.subs);
+  e.sp_count = e.sp_first + nbins;
+  make_list(e.spread_max_per_sub, e.sp_first, e.sp_count, &e.sp_subs, &e.sp_subs_cap, &e.sp_nsub);
   NFFT_HIP_CHECK(R, hipGetLastError());
 }
 

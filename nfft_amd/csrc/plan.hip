@@ -131,6 +131,30 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     geom.E[t] = t < d ? bs[t] + 2 * m + 1 : 1;
   }
   max_per_sub = d == 3 ? 256 : 512;
+  spread_max_per_sub = d == 3 ? 2048 : 4096;
+  // B^T writes its LDS tiles to scratch and reduces them without atomics
+  // when the tile fits LDS and never wraps onto itself (E_t <= n_t)
+  {
+    size_t tb = sizeof(cplx<R>);
+    bool fits = true;
+    for (int t = 0; t < d; t++) {
+      tb *= (size_t)geom.E[t];
+      fits = fits && geom.E[t] <= n[t];
+    }
+    tile_scratch = fits && tb <= 64 * 1024;
+  }
+  auto variant = [](const char *name, int dflt) {
+    const char *v = std::getenv(name);
+    if (!v) return dflt;
+    return std::strcmp(v, "priv") == 0   ? 2
+           : std::strcmp(v, "wave") == 0 ? 1
+           : std::strcmp(v, "lane") == 0 ? 0
+                                         : dflt;
+  };
+  interp_variant = variant("NFFT_HIP_INTERP", 0);
+  // float: private-tile wavefronts (plain FP32 read-modify-write); double:
+  // wavefronts sharing a tile with ds_add_f64
+  spread_variant = variant("NFFT_HIP_SPREAD", sizeof(R) == 4 ? 2 : 1);
 
   // FFT axis passes
   const int cap = sizeof(R) == 4 ? 4096 : 2048;  // elements per workgroup
@@ -175,6 +199,9 @@ void Engine<R>::destroy() {
   hipFree(frac);
   hipFree(perm);
   hipFree(subs);
+  hipFree(sp_subs);
+  hipFree(sp_first);
+  hipFree(tiles);
   hipFree(scratch_i);
   for (auto &e : ev)
     if (e) hipEventDestroy(e);
@@ -192,7 +219,19 @@ void Engine<R>::upload_nodes(const R *x_host) {
 
 template <typename R>
 void Engine<R>::build_nodes() {
-  if (!direct_only) bin_nodes<R>(*this);
+  if (!direct_only) {
+    bin_nodes<R>(*this);
+    if (tile_scratch) {
+      long tot = 1;
+      for (int t = 0; t < d; t++) tot *= geom.E[t];
+      const long need = sp_nsub * tot;
+      if (need > tiles_cap) {
+        if (tiles) NFFT_HIP_CHECK(R, hipFree(tiles));
+        NFFT_HIP_CHECK(R, hipMalloc(&tiles, sizeof(cplx<R>) * (size_t)(need > 0 ? need : 1)));
+        tiles_cap = need;
+      }
+    }
+  }
   nodes_ready = true;
 }
 
@@ -202,7 +241,7 @@ template <typename R>
 hipEvent_t *Engine<R>::call_events(int kind) {
   if (deferred) {
     if (log_n == log_ev.size()) {
-      std::array<hipEvent_t, 4> set;
+      std::array<hipEvent_t, 5> set;
       for (auto &e : set) NFFT_HIP_CHECK(R, hipEventCreate(&e));
       log_ev.push_back(set);
       log_kind.push_back(0);
@@ -213,6 +252,8 @@ hipEvent_t *Engine<R>::call_events(int kind) {
   return timing ? ev : nullptr;
 }
 
+// Event layout of one call.  trafo: E0 | D | E1 | FFT | E2 | B | E3 (E4 = E3).
+// adjoint: E0 | B^T spreading | E1 | tile reduction | E2 | FFT | E3 | D^T | E4.
 template <typename R>
 void Engine<R>::trafo(const cplx<R> *fhat_dev, cplx<R> *f_dev) {
   hipEvent_t *E = call_events(0);
@@ -224,20 +265,26 @@ void Engine<R>::trafo(const cplx<R> *fhat_dev, cplx<R> *f_dev) {
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[2], stream));
   launch_interp<R>(g, f_dev, *this, stream);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[3], stream));
+  if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[4], stream));
   NFFT_HIP_CHECK(R, hipGetLastError());
 }
 
 template <typename R>
 void Engine<R>::adjoint(const cplx<R> *f_dev, cplx<R> *fhat_dev) {
   hipEvent_t *E = call_events(1);
-  launch_fill_zero<R>(g, n_total, stream);
+  // B^T: spreading kernel, then (scratch path) the tile reduction, which
+  // writes every grid cell; the atomic path needs a zeroed grid instead
+  const bool scratch = tile_scratch && sp_nsub > 0;
+  if (!scratch) launch_fill_zero<R>(g, n_total, stream);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[0], stream));
   launch_spread<R>(f_dev, g, *this, stream);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[1], stream));
-  for (auto &ax : axes) launch_fft_axis<R>(g, ax, +1, stream);
+  if (scratch) launch_gather_tiles<R>(*this, g, stream);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[2], stream));
-  launch_deconv_from_grid<R>(g, fhat_dev, phi_inv, d, N, n, stream);
+  for (auto &ax : axes) launch_fft_axis<R>(g, ax, +1, stream);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[3], stream));
+  launch_deconv_from_grid<R>(g, fhat_dev, phi_inv, d, N, n, stream);
+  if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[4], stream));
   NFFT_HIP_CHECK(R, hipGetLastError());
 }
 
@@ -249,15 +296,21 @@ void Engine<R>::timing_begin() {
 
 template <typename R>
 void Engine<R>::timing_end(double *out) {
-  for (int i = 0; i < 8; i++) out[i] = 0;
+  // out: n_trafo, n_adjoint, trafo D, FFT, B, adjoint spreading, tile
+  // reduction, FFT, D^T -- summed seconds
+  for (int i = 0; i < 9; i++) out[i] = 0;
   NFFT_HIP_CHECK(R, hipStreamSynchronize(stream));
   for (size_t c = 0; c < log_n; c++) {
-    float ms[3];
-    for (int i = 0; i < 3; i++)
+    float ms[4];
+    for (int i = 0; i < 4; i++)
       NFFT_HIP_CHECK(R, hipEventElapsedTime(&ms[i], log_ev[c][i], log_ev[c][i + 1]));
-    const int k = log_kind[c];
-    out[k] += 1;
-    for (int i = 0; i < 3; i++) out[2 + 3 * k + i] += ms[i] * 1e-3;
+    if (log_kind[c] == 0) {
+      out[0] += 1;
+      for (int i = 0; i < 3; i++) out[2 + i] += ms[i] * 1e-3;
+    } else {
+      out[1] += 1;
+      for (int i = 0; i < 4; i++) out[5 + i] += ms[i] * 1e-3;
+    }
   }
   deferred = false;
   log_n = 0;
@@ -277,13 +330,13 @@ void Engine<R>::adjoint_direct(const cplx<R> *f_dev, cplx<R> *fhat_dev) {
 
 // stage durations of the last call, in seconds: [0] D, [1] FFT, [2] B
 template <typename R>
-void Engine<R>::stage_times(double out[3]) {
-  float ms[3] = {0, 0, 0};
-  NFFT_HIP_CHECK(R, hipEventSynchronize(ev[3]));
-  for (int i = 0; i < 3; i++) NFFT_HIP_CHECK(R, hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
-  // trafo records D, FFT, B in that order; adjoint B, FFT, D -- the caller
-  // maps the intervals onto MEASURE_TIME_t
-  for (int i = 0; i < 3; i++) out[i] = ms[i] * 1e-3;
+void Engine<R>::stage_times(double out[4]) {
+  float ms[4] = {0, 0, 0, 0};
+  NFFT_HIP_CHECK(R, hipEventSynchronize(ev[4]));
+  for (int i = 0; i < 4; i++) NFFT_HIP_CHECK(R, hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+  // the four intervals of the event layout above; the caller maps them
+  // onto MEASURE_TIME_t
+  for (int i = 0; i < 4; i++) out[i] = ms[i] * 1e-3;
 }
 
 template struct Engine<float>;
@@ -480,12 +533,12 @@ struct PlanApi {
 
   static void store_times(PLAN *p, Engine<R> *e, bool is_adjoint) {
     if (!e->timing) return;
-    double t[3];
+    double t[4];
     e->stage_times(t);
-    if (is_adjoint) {  // recorded order B, FFT, D
-      p->MEASURE_TIME_t[0] = (R)t[2];
-      p->MEASURE_TIME_t[1] = (R)t[1];
-      p->MEASURE_TIME_t[2] = (R)t[0];
+    if (is_adjoint) {  // recorded order: spreading, tile reduction, FFT, D^T
+      p->MEASURE_TIME_t[0] = (R)t[3];
+      p->MEASURE_TIME_t[1] = (R)t[2];
+      p->MEASURE_TIME_t[2] = (R)(t[0] + t[1]);
     } else {
       p->MEASURE_TIME_t[0] = (R)t[0];
       p->MEASURE_TIME_t[1] = (R)t[1];

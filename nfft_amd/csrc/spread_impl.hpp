@@ -67,15 +67,66 @@ __device__ __forceinline__ long region_to_grid(int q, const int (&lo)[D], const 
 }
 
 template <typename R>
-__device__ __forceinline__ void lds_add(cplx<R> *a, R vr, R vi) {
-  atomicAdd(&a->x, vr);
-  atomicAdd(&a->y, vi);
-}
-
-template <typename R>
 __device__ __forceinline__ void global_add(cplx<R> *a, R vr, R vi) {
   unsafeAtomicAdd(&a->x, vr);
   unsafeAtomicAdd(&a->y, vi);
+}
+
+// LDS accumulator of B^T.  On this GPU a float atomic add to LDS (ds_add_f32)
+// is about 40x slower than a 32-bit integer one and 10x slower than a
+// double one, so the float family accumulates in 32-bit fixed point:
+// contributions are scaled by 2^30 / S, S = sum over the work item's nodes of
+// |Re f_j| + |Im f_j|, which bounds every tile cell because the normalised
+// window product never exceeds 1; integer sums are exact and wrap-safe, and
+// the rounding error per contribution is below S / 2^31.  The double family
+// adds in FP64 (ds_add_f64).
+template <typename R> struct LdsAcc;
+template <> struct LdsAcc<float> {
+  using elem = int2;
+  float scale = 1.f, inv_scale = 1.f;
+  __device__ void set_bound(float S) {
+    scale = S > 0.f ? 1073741824.f / S : 1.f;
+    inv_scale = 1.f / scale;
+  }
+  __device__ __forceinline__ void add(elem *a, float vr, float vi) const {
+    atomicAdd(&a->x, __float2int_rn(vr * scale));
+    atomicAdd(&a->y, __float2int_rn(vi * scale));
+  }
+  __device__ __forceinline__ cplx<float> load(const elem *a) const {
+    const int2 v = *a;
+    return cplx<float>{(float)v.x * inv_scale, (float)v.y * inv_scale};
+  }
+  __device__ static elem zero() { return make_int2(0, 0); }
+};
+template <> struct LdsAcc<double> {
+  using elem = double2;
+  __device__ void set_bound(double) {}
+  __device__ __forceinline__ void add(elem *a, double vr, double vi) const {
+    atomicAdd(&a->x, vr);
+    atomicAdd(&a->y, vi);
+  }
+  __device__ __forceinline__ cplx<double> load(const elem *a) const { return *a; }
+  __device__ static elem zero() { return make_double2(0.0, 0.0); }
+};
+
+// sum over the work item's nodes of |Re f| + |Im f|, on every thread
+template <typename R>
+__device__ R block_abs_sum(const cplx<R> *__restrict__ fin, const int *__restrict__ perm,
+                           long first, int count) {
+  __shared__ R red[16];
+  R s = 0;
+  for (int i = threadIdx.x; i < count; i += blockDim.x) {
+    const cplx<R> v = fin[perm[first + i]];
+    s += fabs(v.x) + fabs(v.y);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  R t = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
+  __syncthreads();
+  return t;
 }
 
 // ===================================================================== static W
@@ -102,7 +153,7 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
                                                      const int *__restrict__ cell,
                                                      const R *__restrict__ frac,
                                                      const int4 *__restrict__ subs, BinGeom gg,
-                                                     WinParams<R> wp) {
+                                                     WinParams<R> wp, cplx<R> *__restrict__ tiles) {
   extern __shared__ unsigned char b_smem[];
   cplx<R> *tile = reinterpret_cast<cplx<R> *>(b_smem);
   const int4 sp = subs[blockIdx.x];
@@ -176,16 +227,19 @@ __global__ __launch_bounds__(256) void k_spread_tile(const cplx<R> *__restrict__
                                                      const int *__restrict__ cell,
                                                      const R *__restrict__ frac,
                                                      const int4 *__restrict__ subs, BinGeom gg,
-                                                     WinParams<R> wp) {
+                                                     WinParams<R> wp, cplx<R> *__restrict__ tiles) {
   extern __shared__ unsigned char b_smem[];
-  cplx<R> *tile = reinterpret_cast<cplx<R> *>(b_smem);
+  using Acc = LdsAcc<R>;
+  typename Acc::elem *tile = reinterpret_cast<typename Acc::elem *>(b_smem);
   const int4 sp = subs[blockIdx.x];
   int lo[D];
   bin_corner<D>(sp.x, gg, lo);
   int tot = 1;
 #pragma unroll
   for (int t = 0; t < D; t++) tot *= gg.E[t];
-  for (int q = threadIdx.x; q < tot; q += blockDim.x) tile[q] = cplx<R>{R(0), R(0)};
+  Acc acc;
+  acc.set_bound(block_abs_sum<R>(fin, perm, sp.y, sp.z));
+  for (int q = threadIdx.x; q < tot; q += blockDim.x) tile[q] = Acc::zero();
   const int E1 = D >= 2 ? gg.E[1] : 1;
   const int E2 = D == 3 ? gg.E[2] : 1;
   __syncthreads();
@@ -196,36 +250,288 @@ __global__ __launch_bounds__(256) void k_spread_tile(const cplx<R> *__restrict__
     node_window<R, D, W>(cell, frac, p, gg.M, wp, c, psi);
     const cplx<R> fj = fin[perm[p]];
     if constexpr (D == 1) {
-      cplx<R> *row = tile + (c[0] - lo[0]);
+      typename Acc::elem *row = tile + (c[0] - lo[0]);
 #pragma unroll
-      for (int w = 0; w < W; w++) lds_add<R>(row + w, psi[0][w] * fj.x, psi[0][w] * fj.y);
+      for (int w = 0; w < W; w++) acc.add(row + w, psi[0][w] * fj.x, psi[0][w] * fj.y);
     } else if constexpr (D == 2) {
-      cplx<R> *base = tile + (c[0] - lo[0]) * E1 + (c[1] - lo[1]);
+      typename Acc::elem *base = tile + (c[0] - lo[0]) * E1 + (c[1] - lo[1]);
 #pragma unroll
       for (int i0 = 0; i0 < W; i0++) {
-        cplx<R> *row = base + i0 * E1;
+        typename Acc::elem *row = base + i0 * E1;
         const R wr = psi[0][i0] * fj.x, wi = psi[0][i0] * fj.y;
 #pragma unroll
-        for (int w = 0; w < W; w++) lds_add<R>(row + w, psi[1][w] * wr, psi[1][w] * wi);
+        for (int w = 0; w < W; w++) acc.add(row + w, psi[1][w] * wr, psi[1][w] * wi);
       }
     } else {
-      cplx<R> *base = tile + ((c[0] - lo[0]) * E1 + (c[1] - lo[1])) * E2 + (c[2] - lo[2]);
+      typename Acc::elem *base = tile + ((c[0] - lo[0]) * E1 + (c[1] - lo[1])) * E2 + (c[2] - lo[2]);
       for (int i0 = 0; i0 < W; i0++) {
         const R wr0 = psi[0][i0] * fj.x, wi0 = psi[0][i0] * fj.y;
 #pragma unroll
         for (int i1 = 0; i1 < W; i1++) {
-          cplx<R> *row = base + (i0 * E1 + i1) * E2;
+          typename Acc::elem *row = base + (i0 * E1 + i1) * E2;
           const R wr = psi[1][i1] * wr0, wi = psi[1][i1] * wi0;
 #pragma unroll
-          for (int w = 0; w < W; w++) lds_add<R>(row + w, psi[2][w] * wr, psi[2][w] * wi);
+          for (int w = 0; w < W; w++) acc.add(row + w, psi[2][w] * wr, psi[2][w] * wi);
         }
       }
     }
   }
   __syncthreads();
   for (int q = threadIdx.x; q < tot; q += blockDim.x) {
+    const cplx<R> v = acc.load(tile + q);
+    if (tiles) tiles[(long)blockIdx.x * tot + q] = v;
+    else if (v.x != R(0) || v.y != R(0)) global_add<R>(g + region_to_grid<D>(q, lo, gg), v.x, v.y);
+  }
+}
+
+// ============================================================ wave per node
+// d = 2, 3: one wavefront handles one node at a time.  Lanes own fixed pairs
+// of tap indices over the last two dimensions (W*W pairs in ceil(W*W/64)
+// slots); for d = 3 the first dimension is a loop whose weight is read from
+// one lane (readlane, uniform).  The d*W window values of the node are
+// computed once, one per lane, and fetched with ds_bpermute.  In one wave
+// instruction all lanes touch distinct taps, so LDS atomics of a dense
+// cluster (many nodes in one cell) never collide inside the wave, and the
+// window is evaluated d*W times per node instead of per lane.
+template <typename R>
+__device__ __forceinline__ R lane_bcast(R v, int src) {
+  if constexpr (sizeof(R) == 4) {
+    return __builtin_bit_cast(R, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
+  } else {
+    long long u = __builtin_bit_cast(long long, v);
+    int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffLL), src);
+    int hi = __builtin_amdgcn_readlane((int)(u >> 32), src);
+    return __builtin_bit_cast(R, ((long long)hi << 32) | (unsigned int)lo);
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ R lane_fetch(R v, int src) {
+  return __shfl(v, src, 64);
+}
+
+template <typename R>
+__device__ __forceinline__ R wave_sum(R v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename R, int D, int W, bool SPREAD>
+__global__ __launch_bounds__(256) void k_bwave_tile(const cplx<R> *__restrict__ src,
+                                                    cplx<R> *__restrict__ dst,
+                                                    const int *__restrict__ perm,
+                                                    const int *__restrict__ cell,
+                                                    const R *__restrict__ frac,
+                                                    const int4 *__restrict__ subs, BinGeom gg,
+                                                    WinParams<R> wp, cplx<R> *__restrict__ tiles) {
+  static_assert(D == 2 || D == 3, "wave-per-node kernels cover d = 2, 3");
+  constexpr int m = W / 2 - 1;
+  constexpr int PAIRS = W * W;
+  constexpr int SLOTS = (PAIRS + 63) / 64;
+  extern __shared__ unsigned char b_smem[];
+  cplx<R> *tile = reinterpret_cast<cplx<R> *>(b_smem);
+  using Acc = LdsAcc<R>;
+  typename Acc::elem *atile = reinterpret_cast<typename Acc::elem *>(b_smem);
+  Acc acc;
+  const int4 sp = subs[blockIdx.x];
+  int lo[D];
+  bin_corner<D>(sp.x, gg, lo);
+  int tot = 1;
+#pragma unroll
+  for (int t = 0; t < D; t++) tot *= gg.E[t];
+  if constexpr (SPREAD) {
+    acc.set_bound(block_abs_sum<R>(src, perm, sp.y, sp.z));
+    for (int q = threadIdx.x; q < tot; q += blockDim.x) atile[q] = Acc::zero();
+  } else {
+    for (int q = threadIdx.x; q < tot; q += blockDim.x) tile[q] = src[region_to_grid<D>(q, lo, gg)];
+  }
+  __syncthreads();
+  const int E1 = gg.E[1];
+  const int E2 = D == 3 ? gg.E[2] : 1;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = blockDim.x >> 6;
+  // this lane's tap pairs (pa over dim D-2, pb over dim D-1)
+  int pa[SLOTS], pb[SLOTS], off[SLOTS];
+  bool pv[SLOTS];
+#pragma unroll
+  for (int s = 0; s < SLOTS; s++) {
+    const int q = lane + 64 * s;
+    pv[s] = q < PAIRS;
+    pa[s] = pv[s] ? q / W : 0;
+    pb[s] = pv[s] ? q - (q / W) * W : 0;
+    off[s] = D == 3 ? pa[s] * E2 + pb[s] : pa[s] * E1 + pb[s];
+  }
+  // window value owned by this lane: dimension lane / W, tap lane % W
+  const int wt = lane / W < D ? lane / W : D - 1;
+  const int ww = lane - (lane / W) * W;
+  for (int i = wv; i < sp.z; i += nw) {
+    const long p = (long)sp.y + i;
+    R mine = R(0);
+    if (lane < D * W) mine = kb_psi<R>(frac[wt * gg.M + p] + R(m - ww), R(m), wp.b[wt], wp.inv_den[wt]);
+    int a[D];
+#pragma unroll
+    for (int t = 0; t < D; t++) a[t] = cell[t * gg.M + p] - lo[t];
+    // product weights of this lane's pairs
+    R wpair[SLOTS];
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+      const R u = lane_fetch<R>(mine, (D - 2) * W + pa[s]);
+      const R v = lane_fetch<R>(mine, (D - 1) * W + pb[s]);
+      wpair[s] = pv[s] ? u * v : R(0);
+    }
+    if constexpr (SPREAD) {
+      const cplx<R> fj = src[perm[p]];
+      if constexpr (D == 2) {
+        typename Acc::elem *base = atile + a[0] * E1 + a[1];
+#pragma unroll
+        for (int s = 0; s < SLOTS; s++)
+          if (pv[s]) acc.add(base + off[s], wpair[s] * fj.x, wpair[s] * fj.y);
+      } else {
+        for (int i0 = 0; i0 < W; i0++) {
+          const R p0 = lane_bcast<R>(mine, i0);
+          const R wr = p0 * fj.x, wi = p0 * fj.y;
+          typename Acc::elem *base = atile + ((a[0] + i0) * E1 + a[1]) * E2 + a[2];
+#pragma unroll
+          for (int s = 0; s < SLOTS; s++)
+            if (pv[s]) acc.add(base + off[s], wpair[s] * wr, wpair[s] * wi);
+        }
+      }
+    } else {
+      R ar = 0, ai = 0;
+      if constexpr (D == 2) {
+        const cplx<R> *base = tile + a[0] * E1 + a[1];
+#pragma unroll
+        for (int s = 0; s < SLOTS; s++)
+          if (pv[s]) {
+            const cplx<R> v = base[off[s]];
+            ar += wpair[s] * v.x;
+            ai += wpair[s] * v.y;
+          }
+      } else {
+        for (int i0 = 0; i0 < W; i0++) {
+          const R p0 = lane_bcast<R>(mine, i0);
+          const cplx<R> *base = tile + ((a[0] + i0) * E1 + a[1]) * E2 + a[2];
+          R sr = 0, si = 0;
+#pragma unroll
+          for (int s = 0; s < SLOTS; s++)
+            if (pv[s]) {
+              const cplx<R> v = base[off[s]];
+              sr += wpair[s] * v.x;
+              si += wpair[s] * v.y;
+            }
+          ar += p0 * sr;
+          ai += p0 * si;
+        }
+      }
+      ar = wave_sum<R>(ar);
+      ai = wave_sum<R>(ai);
+      if (lane == 0) dst[perm[p]] = cplx<R>{ar, ai};
+    }
+  }
+  if constexpr (SPREAD) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < tot; q += blockDim.x) {
+      const cplx<R> v = acc.load(atile + q);
+      if (tiles) tiles[(long)blockIdx.x * tot + q] = v;
+      else if (v.x != R(0) || v.y != R(0)) global_add<R>(dst + region_to_grid<D>(q, lo, gg), v.x, v.y);
+    }
+  }
+}
+
+// ======================================================= private-tile B^T
+// d = 2, 3, one wavefront per work item and an LDS tile of its own: lanes
+// own fixed tap pairs of the last two dimensions (as in k_bwave_tile), so
+// within one instruction every lane touches a different cell, and plain
+// read-modify-write replaces LDS atomics (exact FP32 sums, no ds_add_f32).
+// Per node all W*SLOTS cells of the lane are loaded first, then updated and
+// stored, so the LDS latency is paid once per node.  Instructions of one
+// wave reach LDS in order, so a later node's loads see earlier stores.
+template <typename R, int D, int W>
+__global__ __launch_bounds__(64) void k_spread_wave_priv(const cplx<R> *__restrict__ fin,
+                                                         cplx<R> *__restrict__ g,
+                                                         const int *__restrict__ perm,
+                                                         const int *__restrict__ cell,
+                                                         const R *__restrict__ frac,
+                                                         const int4 *__restrict__ subs,
+                                                         BinGeom gg, WinParams<R> wp,
+                                                         cplx<R> *__restrict__ tiles) {
+  static_assert(D == 2 || D == 3, "private-tile kernels cover d = 2, 3");
+  constexpr int m = W / 2 - 1;
+  constexpr int PAIRS = W * W;
+  constexpr int SLOTS = (PAIRS + 63) / 64;
+  constexpr int ROWS = D == 3 ? W : 1;  // first-dimension taps looped per lane
+  extern __shared__ unsigned char b_smem[];
+  cplx<R> *tile = reinterpret_cast<cplx<R> *>(b_smem);
+  const int4 sp = subs[blockIdx.x];
+  int lo[D];
+  bin_corner<D>(sp.x, gg, lo);
+  int tot = 1;
+#pragma unroll
+  for (int t = 0; t < D; t++) tot *= gg.E[t];
+  const int lane = threadIdx.x;
+  for (int q = lane; q < tot; q += 64) tile[q] = cplx<R>{R(0), R(0)};
+  __syncthreads();
+  const int E1 = gg.E[1];
+  const int E2 = D == 3 ? gg.E[2] : 1;
+  int pa[SLOTS], pb[SLOTS], off[SLOTS];
+  bool pv[SLOTS];
+#pragma unroll
+  for (int s = 0; s < SLOTS; s++) {
+    const int q = lane + 64 * s;
+    pv[s] = q < PAIRS;
+    pa[s] = pv[s] ? q / W : 0;
+    pb[s] = pv[s] ? q - (q / W) * W : 0;
+    off[s] = D == 3 ? pa[s] * E2 + pb[s] : pa[s] * E1 + pb[s];
+  }
+  const int wt = lane / W < D ? lane / W : D - 1;
+  const int ww = lane - (lane / W) * W;
+  const int rowstride = D == 3 ? E1 * E2 : 0;
+  for (int i = 0; i < sp.z; i++) {
+    const long p = (long)sp.y + i;
+    R mine = R(0);
+    if (lane < D * W) mine = kb_psi<R>(frac[wt * gg.M + p] + R(m - ww), R(m), wp.b[wt], wp.inv_den[wt]);
+    int a[D];
+#pragma unroll
+    for (int t = 0; t < D; t++) a[t] = cell[t * gg.M + p] - lo[t];
+    R wpair[SLOTS];
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+      const R u = lane_fetch<R>(mine, (D - 2) * W + pa[s]);
+      const R v = lane_fetch<R>(mine, (D - 1) * W + pb[s]);
+      wpair[s] = pv[s] ? u * v : R(0);
+    }
+    const cplx<R> fj = fin[perm[p]];
+    cplx<R> *base = D == 3 ? tile + ((a[0] * E1 + a[1]) * E2 + a[2]) : tile + (a[0] * E1 + a[1]);
+    // load every cell this lane updates for this node, then update, then store
+    cplx<R> v[ROWS][SLOTS];
+#pragma unroll
+    for (int r = 0; r < ROWS; r++)
+#pragma unroll
+      for (int s = 0; s < SLOTS; s++)
+        v[r][s] = pv[s] ? base[r * rowstride + off[s]] : cplx<R>{R(0), R(0)};
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+      const R p0 = D == 3 ? lane_bcast<R>(mine, r) : R(1);
+      const R wr = p0 * fj.x, wi = p0 * fj.y;
+#pragma unroll
+      for (int s = 0; s < SLOTS; s++) {
+        v[r][s].x += wpair[s] * wr;
+        v[r][s].y += wpair[s] * wi;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; r++)
+#pragma unroll
+      for (int s = 0; s < SLOTS; s++)
+        if (pv[s]) base[r * rowstride + off[s]] = v[r][s];
+  }
+  __syncthreads();
+  for (int q = lane; q < tot; q += 64) {
     const cplx<R> v = tile[q];
-    if (v.x != R(0) || v.y != R(0)) global_add<R>(g + region_to_grid<D>(q, lo, gg), v.x, v.y);
+    if (tiles) tiles[(long)blockIdx.x * tot + q] = v;
+    else if (v.x != R(0) || v.y != R(0)) global_add<R>(g + region_to_grid<D>(q, lo, gg), v.x, v.y);
   }
 }
 
@@ -253,7 +559,7 @@ __global__ __launch_bounds__(256) void k_interp_rt(const cplx<R> *__restrict__ g
                                                    const int *__restrict__ cell,
                                                    const R *__restrict__ frac,
                                                    const int4 *__restrict__ subs, BinGeom gg,
-                                                   WinParams<R> wp) {
+                                                   WinParams<R> wp, cplx<R> *__restrict__ tiles) {
   extern __shared__ unsigned char b_smem[];
   cplx<R> *tile = reinterpret_cast<cplx<R> *>(b_smem);
   const int W = 2 * gg.m + 2;
@@ -334,9 +640,11 @@ __global__ __launch_bounds__(256) void k_spread_rt(const cplx<R> *__restrict__ f
                                                    const int *__restrict__ cell,
                                                    const R *__restrict__ frac,
                                                    const int4 *__restrict__ subs, BinGeom gg,
-                                                   WinParams<R> wp) {
+                                                   WinParams<R> wp, cplx<R> *__restrict__ tiles) {
   extern __shared__ unsigned char b_smem[];
-  cplx<R> *tile = reinterpret_cast<cplx<R> *>(b_smem);
+  using Acc = LdsAcc<R>;
+  typename Acc::elem *tile = reinterpret_cast<typename Acc::elem *>(b_smem);
+  Acc acc;
   const int W = 2 * gg.m + 2;
   const int4 sp = subs[blockIdx.x];
   int lo[D];
@@ -344,7 +652,8 @@ __global__ __launch_bounds__(256) void k_spread_rt(const cplx<R> *__restrict__ f
   int tot = 1;
   if constexpr (TILE) {
     for (int t = 0; t < D; t++) tot *= gg.E[t];
-    for (int q = threadIdx.x; q < tot; q += blockDim.x) tile[q] = cplx<R>{R(0), R(0)};
+    acc.set_bound(block_abs_sum<R>(fin, perm, sp.y, sp.z));
+    for (int q = threadIdx.x; q < tot; q += blockDim.x) tile[q] = Acc::zero();
     __syncthreads();
   }
   for (int i = threadIdx.x; i < sp.z; i += blockDim.x) {
@@ -366,31 +675,26 @@ __global__ __launch_bounds__(256) void k_spread_rt(const cplx<R> *__restrict__ f
         sdim *= TILE ? gg.E[t] : gg.n[t];
       }
     }
-    cplx<R> *dst = TILE ? tile : g;
+    // one tap: LDS accumulator (tile) or HBM atomic (global variant)
+    auto put = [&](long o, R vr, R vi) {
+      if constexpr (TILE) acc.add(tile + o, vr, vi);
+      else global_add<R>(g + o, vr, vi);
+    };
     if constexpr (D == 1) {
-      for (int w = 0; w < W; w++) {
-        if (TILE) lds_add<R>(dst + idx[0][w], psi[0][w] * fj.x, psi[0][w] * fj.y);
-        else global_add<R>(dst + idx[0][w], psi[0][w] * fj.x, psi[0][w] * fj.y);
-      }
+      for (int w = 0; w < W; w++) put(idx[0][w], psi[0][w] * fj.x, psi[0][w] * fj.y);
     } else if constexpr (D == 2) {
       for (int i0 = 0; i0 < W; i0++) {
-        cplx<R> *row = dst + idx[0][i0] * stride[0];
+        const long row = idx[0][i0] * stride[0];
         const R wr = psi[0][i0] * fj.x, wi = psi[0][i0] * fj.y;
-        for (int w = 0; w < W; w++) {
-          if (TILE) lds_add<R>(row + idx[1][w], psi[1][w] * wr, psi[1][w] * wi);
-          else global_add<R>(row + idx[1][w], psi[1][w] * wr, psi[1][w] * wi);
-        }
+        for (int w = 0; w < W; w++) put(row + idx[1][w], psi[1][w] * wr, psi[1][w] * wi);
       }
     } else {
       for (int i0 = 0; i0 < W; i0++) {
         for (int i1 = 0; i1 < W; i1++) {
-          cplx<R> *row = dst + idx[0][i0] * stride[0] + idx[1][i1] * stride[1];
+          const long row = idx[0][i0] * stride[0] + idx[1][i1] * stride[1];
           const R w01 = psi[0][i0] * psi[1][i1];
           const R wr = w01 * fj.x, wi = w01 * fj.y;
-          for (int w = 0; w < W; w++) {
-            if (TILE) lds_add<R>(row + idx[2][w], psi[2][w] * wr, psi[2][w] * wi);
-            else global_add<R>(row + idx[2][w], psi[2][w] * wr, psi[2][w] * wi);
-          }
+          for (int w = 0; w < W; w++) put(row + idx[2][w], psi[2][w] * wr, psi[2][w] * wi);
         }
       }
     }
@@ -398,18 +702,128 @@ __global__ __launch_bounds__(256) void k_spread_rt(const cplx<R> *__restrict__ f
   if constexpr (TILE) {
     __syncthreads();
     for (int q = threadIdx.x; q < tot; q += blockDim.x) {
-      const cplx<R> v = tile[q];
-      if (v.x != R(0) || v.y != R(0)) global_add<R>(g + region_to_grid<D>(q, lo, gg), v.x, v.y);
+      const cplx<R> v = acc.load(tile + q);
+      if (tiles) tiles[(long)blockIdx.x * tot + q] = v;
+      else if (v.x != R(0) || v.y != R(0)) global_add<R>(g + region_to_grid<D>(q, lo, gg), v.x, v.y);
     }
   }
+}
+
+// ============================================================ tile reduction
+// Second half of the atomic-free B^T: one workgroup per bin writes every
+// grid cell of the bin's core (bs_t cells per dimension, clipped at the grid
+// edge) as the sum of the spreading tiles that cover it -- the tiles of this
+// bin and of the neighbour bins within 1 + ceil(m / bs_t) bins, all of their
+// work items.  Every grid cell is written exactly once, so the grid needs no
+// zero fill and no atomics.  Requires E_t <= n_t (a tile never covers a cell
+// twice); plans with smaller grids use the atomic flush instead.
+constexpr int kMaxSpan = 40;
+
+template <typename R, int D>
+__global__ __launch_bounds__(256) void k_gather_tiles(const cplx<R> *__restrict__ tiles,
+                                                      const int *__restrict__ sp_first,
+                                                      const int *__restrict__ sp_count,
+                                                      cplx<R> *__restrict__ g, BinGeom gg) {
+  // per dimension: neighbour bin coordinates and the tile coordinate, in
+  // that neighbour's tile, of this bin's first core cell
+  __shared__ int nb_bt[3][kMaxSpan];
+  __shared__ int nb_e[3][kMaxSpan];
+  __shared__ int nb_span[3];
+  int bc[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, cs[3] = {1, 1, 1}, E[3] = {1, 1, 1}, n[3] = {1, 1, 1},
+      nbv[3] = {1, 1, 1};
+  {
+    int r = blockIdx.x;
+#pragma unroll
+    for (int t = D - 1; t >= 0; t--) {
+      bc[t] = r % gg.nb[t];
+      r /= gg.nb[t];
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < D; t++) {
+    lo[t] = bc[t] * gg.bs[t];
+    cs[t] = min(gg.bs[t], gg.n[t] - lo[t]);
+    E[t] = gg.E[t];
+    n[t] = gg.n[t];
+    nbv[t] = gg.nb[t];
+  }
+  if (threadIdx.x < 3) {
+    const int t = threadIdx.x;
+    if (t >= D) {
+      nb_span[t] = 1;
+      nb_bt[t][0] = 0;
+      nb_e[t][0] = 0;
+    } else {
+      const int reach = 1 + (gg.m + gg.bs[t] - 1) / gg.bs[t];
+      const bool all = 2 * reach + 1 >= gg.nb[t];
+      const int span = all ? gg.nb[t] : 2 * reach + 1;
+      nb_span[t] = span;
+      for (int j = 0; j < span && j < kMaxSpan; j++) {
+        const int bt = all ? j : ((bc[t] + j - reach) % gg.nb[t] + gg.nb[t]) % gg.nb[t];
+        int e = (lo[t] - bt * gg.bs[t] + gg.m) % gg.n[t];
+        nb_bt[t][j] = bt;
+        nb_e[t][j] = e < 0 ? e + gg.n[t] : e;
+      }
+    }
+  }
+  __syncthreads();
+  const int s0 = nb_span[0], s1 = nb_span[1], s2 = nb_span[2];
+  const long tot = (long)E[0] * E[1] * E[2];
+  const int core = cs[0] * cs[1] * cs[2];
+  for (int q = threadIdx.x; q < core; q += blockDim.x) {
+    const int k2 = q % cs[2];
+    const int k1 = (q / cs[2]) % cs[1];
+    const int k0 = q / (cs[2] * cs[1]);
+    R sr = 0, si = 0;
+    for (int j0 = 0; j0 < s0; j0++) {
+      int e0 = nb_e[0][j0] + k0;
+      e0 -= e0 >= n[0] ? n[0] : 0;
+      if (e0 >= E[0]) continue;
+      for (int j1 = 0; j1 < s1; j1++) {
+        int e1 = nb_e[1][j1] + k1;
+        e1 -= e1 >= n[1] ? n[1] : 0;
+        if (e1 >= E[1]) continue;
+        const int b01 = nb_bt[0][j0] * nbv[1] + nb_bt[1][j1];
+        const long t01 = (long)e0 * E[1] + e1;
+        for (int j2 = 0; j2 < s2; j2++) {
+          int e2 = nb_e[2][j2] + k2;
+          e2 -= e2 >= n[2] ? n[2] : 0;
+          if (e2 >= E[2]) continue;
+          const int nbl = b01 * nbv[2] + nb_bt[2][j2];
+          const int cnt = sp_count[nbl];
+          const cplx<R> *tp = tiles + (long)sp_first[nbl] * tot + t01 * E[2] + e2;
+          for (int c = 0; c < cnt; c++) {
+            const cplx<R> v = tp[(long)c * tot];
+            sr += v.x;
+            si += v.y;
+          }
+        }
+      }
+    }
+    const long gi = ((long)(lo[0] + k0) * n[1] + (lo[1] + k1)) * n[2] + (lo[2] + k2);
+    g[gi] = cplx<R>{sr, si};
+  }
+}
+
+template <typename R>
+void launch_gather_tiles(const Engine<R> &e, cplx<R> *g, hipStream_t s) {
+  long nbins = 1;
+  for (int t = 0; t < e.geom.d; t++) nbins *= e.geom.nb[t];
+  auto fn = e.geom.d == 1 ? k_gather_tiles<R, 1> : e.geom.d == 2 ? k_gather_tiles<R, 2>
+                                                                 : k_gather_tiles<R, 3>;
+  hipLaunchKernelGGL(fn, dim3((unsigned)nbins), dim3(256), 0, s, e.tiles, e.sp_first, e.sp_count,
+                     g, e.geom);
 }
 
 // ==================================================================== dispatch
 template <typename R> struct BKernelTable {
   using fn = void (*)(const cplx<R> *, cplx<R> *, const int *, const int *, const R *,
-                      const int4 *, BinGeom, WinParams<R>);
+                      const int4 *, BinGeom, WinParams<R>, cplx<R> *);
   fn interp_static[kMaxDim][kMaxStaticM + 1];
   fn spread_static[kMaxDim][kMaxStaticM + 1];
+  fn interp_wave[kMaxDim][kMaxStaticM + 1];  // d = 2, 3 only
+  fn spread_wave[kMaxDim][kMaxStaticM + 1];
+  fn spread_priv[kMaxDim][kMaxStaticM + 1];  // d = 2, 3 only, 64 threads
   fn interp_rt[kMaxDim][2];  // [d-1][tile]
   fn spread_rt[kMaxDim][2];
 };
@@ -418,6 +832,11 @@ template <typename R, int D, int M_>
 void bk_fill_one(BKernelTable<R> &k) {
   k.interp_static[D - 1][M_] = k_interp_tile<R, D, 2 * M_ + 2>;
   k.spread_static[D - 1][M_] = k_spread_tile<R, D, 2 * M_ + 2>;
+  if constexpr (D >= 2 && D * (2 * M_ + 2) <= 64) {
+    k.interp_wave[D - 1][M_] = k_bwave_tile<R, D, 2 * M_ + 2, false>;
+    k.spread_wave[D - 1][M_] = k_bwave_tile<R, D, 2 * M_ + 2, true>;
+    k.spread_priv[D - 1][M_] = k_spread_wave_priv<R, D, 2 * M_ + 2>;
+  }
 }
 
 template <typename R, int D, int... Ms>
@@ -449,14 +868,27 @@ size_t tile_bytes(const BinGeom &gg) {
 // LDS tiles above this size use the global-memory variant
 constexpr size_t kTileBudget = 64 * 1024;
 
+// variant: 0 lane per node, 1 wave per node, 2 wave per node with a private
+// tile (spreading only); falls back to what exists for (d, m)
 template <typename R>
 typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeom &gg,
-                                          bool spread, size_t &lds) {
+                                          bool spread, int variant, size_t &lds, int &threads) {
   const size_t bytes = tile_bytes<R>(gg);
   const bool tile = bytes <= kTileBudget;
   lds = tile ? bytes : 0;
-  if (tile && gg.m >= 1 && gg.m <= kMaxStaticM)
-    return spread ? k.spread_static[gg.d - 1][gg.m] : k.interp_static[gg.d - 1][gg.m];
+  threads = 256;
+  if (tile && gg.m >= 1 && gg.m <= kMaxStaticM) {
+    const int t = gg.d - 1;
+    if (spread && variant == 2 && k.spread_priv[t][gg.m]) {
+      threads = 64;
+      return k.spread_priv[t][gg.m];
+    }
+    if (variant >= 1) {
+      auto fn = spread ? k.spread_wave[t][gg.m] : k.interp_wave[t][gg.m];
+      if (fn) return fn;
+    }
+    return spread ? k.spread_static[t][gg.m] : k.interp_static[t][gg.m];
+  }
   return spread ? k.spread_rt[gg.d - 1][tile ? 1 : 0] : k.interp_rt[gg.d - 1][tile ? 1 : 0];
 }
 
@@ -470,18 +902,22 @@ template <typename R>
 void launch_interp(const cplx<R> *g, cplx<R> *f, const Engine<R> &e, hipStream_t s) {
   if (e.nsub == 0) return;
   size_t lds;
-  auto fn = pick_bkernel<R>(bkernels<R>(), e.geom, false, lds);
-  hipLaunchKernelGGL(fn, dim3((unsigned)e.nsub), dim3(256), lds, s, g, f, e.perm, e.cell, e.frac,
-                     e.subs, e.geom, e.win);
+  int threads;
+  auto fn = pick_bkernel<R>(bkernels<R>(), e.geom, false, e.interp_variant, lds, threads);
+  hipLaunchKernelGGL(fn, dim3((unsigned)e.nsub), dim3(threads), lds, s, g, f, e.perm, e.cell, e.frac,
+                     e.subs, e.geom, e.win, (cplx<R> *)nullptr);
 }
 
 template <typename R>
 void launch_spread(const cplx<R> *f, cplx<R> *g, const Engine<R> &e, hipStream_t s) {
-  if (e.nsub == 0) return;
+  if (e.sp_nsub == 0) return;
   size_t lds;
-  auto fn = pick_bkernel<R>(bkernels<R>(), e.geom, true, lds);
-  hipLaunchKernelGGL(fn, dim3((unsigned)e.nsub), dim3(256), lds, s, f, g, e.perm, e.cell, e.frac,
-                     e.subs, e.geom, e.win);
+  int threads;
+  auto fn = pick_bkernel<R>(bkernels<R>(), e.geom, true, e.spread_variant, lds, threads);
+  // the scratch reduction needs an LDS tile; the global variant adds directly
+  cplx<R> *tiles = (lds > 0 && e.tile_scratch) ? e.tiles : nullptr;
+  hipLaunchKernelGGL(fn, dim3((unsigned)e.sp_nsub), dim3(threads), lds, s, f, g, e.perm, e.cell,
+                     e.frac, e.sp_subs, e.geom, e.win, tiles);
 }
 
 }  // namespace nfft_hip

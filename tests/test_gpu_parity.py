@@ -93,6 +93,23 @@ def test_vs_oracle(case, precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("variant", ["lane", "wave", "priv"])
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("case", [c for c in CASES if len(c[1]) >= 2], ids=[c[0] for c in CASES if len(c[1]) >= 2])
+def test_b_kernel_variants(case, precision, variant, monkeypatch):
+    """Both shapes of the B / B^T kernels (lane per node, wave per node)."""
+    monkeypatch.setenv("NFFT_HIP_INTERP", variant)
+    monkeypatch.setenv("NFFT_HIP_SPREAD", variant)
+    name, N, gen, m = case
+    x = gen()
+    fh = W.complex_normal(int(np.prod(N)), seed=23)
+    f = W.complex_normal(x.shape[0], seed=24)
+    (fg, fr, _), (hg, hr, _) = run_pair(N, x, fh, f, m, precision)
+    tol = 1e-12 if precision == "double" else 1e-5
+    assert rel_errs(fg, fr)[0] < tol and rel_errs(hg, hr)[0] < tol
+
+
+@pytest.mark.gpu
 def test_cfg1_vs_ndft():
     """1-D N=1024, M=4096, sigma=2, m=6, FP64 against the direct NDFT under the
     reference's bound (tests/nfft.c:217-284)."""
