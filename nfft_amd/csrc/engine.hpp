@@ -37,7 +37,15 @@ struct BinGeom {
 template <typename R> struct WinParams {
   R b[kMaxDim];        // shape parameter pi (2 - 1/sigma_t)
   R inv_den[kMaxDim];  // 1 / (1 - exp(-2 b m))
+  R tail[kMaxDim];     // 2 exp(-b m) / (1 - exp(-2 b m))
+  // Per dimension t and tap w, the scaled window psi(frac + m - w) as a
+  // degree kPolyDeg polynomial in (frac - 1/2), coefficients lowest first at
+  // poly[(t * (2m+2) + w) * kPolyCoef + k]; fitted on the host in long double
+  // (Chebyshev interpolation, |error| < 1e-9 of the peak for m <= 8).
+  const R *poly;
 };
+constexpr int kPolyDeg = 10;
+constexpr int kPolyCoef = kPolyDeg + 1;
 
 // One axis pass of the oversampled FFT: the grid viewed as
 // [outer][len][inner]; each line (outer o, inner i) is transformed.
@@ -75,6 +83,7 @@ template <typename R> struct Engine {
   cplx<R> *f = nullptr;
   R *xdev = nullptr;
   R *phi_inv = nullptr;  // sum_t N_t scaled deconvolution factors
+  R *poly = nullptr;     // window polynomials (WinParams::poly)
   int *cell = nullptr;
   R *frac = nullptr;
   int *perm = nullptr;

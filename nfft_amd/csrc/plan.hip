@@ -42,6 +42,62 @@ static long double phi_hut(long n, long k, int m, long double b) {
   return bessel_i0((long double)m * sqrtl(b * b - w * w));
 }
 
+// The window of include/infft.h:209-215 divided by its peak, in long double:
+// sinh(b s)/(pi s) for s = sqrt(m^2 - y^2) > 0, sin(b s)/(pi s) beyond |y| = m.
+static long double psi_scaled(long double y, int m, long double b) {
+  const long double q = (long double)m * m - y * y;
+  const long double peak = sinhl(b * m) / (kPiL * m);
+  long double v;
+  if (q > 0) v = sinhl(b * sqrtl(q)) / (kPiL * sqrtl(q));
+  else if (q < 0) v = sinl(b * sqrtl(-q)) / (kPiL * sqrtl(-q));
+  else v = b / kPiL;
+  return v / peak;
+}
+
+// Per tap w = 0..2m+1 of one dimension, psi(frac + m - w) for frac in [0,1)
+// as a polynomial in t = frac - 1/2 of degree kPolyDeg: Chebyshev
+// interpolation at 32 points, converted to monomials in t.  The window is an
+// entire function of y^2, so the fit converges geometrically.
+template <typename R>
+static void fit_window_polys(int m, long double b, std::vector<R> &out) {
+  const int K = 32, DEG = kPolyDeg;
+  for (int w = 0; w < 2 * m + 2; w++) {
+    long double f[K], u[K];
+    for (int j = 0; j < K; j++) {
+      u[j] = cosl(kPiL * (j + 0.5L) / K);  // u = 2t in [-1, 1]
+      f[j] = psi_scaled(0.5L * u[j] + 0.5L + m - w, m, b);
+    }
+    long double c[DEG + 1];
+    for (int k = 0; k <= DEG; k++) {
+      long double s = 0;
+      for (int j = 0; j < K; j++) s += f[j] * cosl(k * kPiL * (j + 0.5L) / K);
+      c[k] = s * 2.0L / K;
+    }
+    c[0] *= 0.5L;
+    // Chebyshev series in u -> monomials in u (T_{k+1} = 2u T_k - T_{k-1})
+    long double mono[DEG + 1] = {0}, tkm1[DEG + 1] = {0}, tk[DEG + 1] = {0};
+    tkm1[0] = 1;  // T0
+    tk[1] = 1;    // T1
+    for (int i = 0; i <= DEG; i++) mono[i] += c[0] * tkm1[i];
+    if (DEG >= 1)
+      for (int i = 0; i <= DEG; i++) mono[i] += c[1] * tk[i];
+    for (int k = 2; k <= DEG; k++) {
+      long double tn[DEG + 1] = {0};
+      for (int i = 0; i <= DEG; i++) {
+        if (i > 0) tn[i] += 2 * tk[i - 1];
+        tn[i] -= tkm1[i];
+      }
+      for (int i = 0; i <= DEG; i++) {
+        mono[i] += c[k] * tn[i];
+        tkm1[i] = tk[i];
+        tk[i] = tn[i];
+      }
+    }
+    // u = 2t: coefficient of t^i is mono[i] 2^i
+    for (int i = 0; i <= DEG; i++) out.push_back((R)(mono[i] * ldexpl(1.0L, i)));
+  }
+}
+
 // factor n into FFT radices, largest first; returns false if a prime > 13 remains
 static bool fft_radices(long n, std::vector<int> &out) {
   out.clear();
@@ -99,10 +155,22 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     long double peak = sinhl(b * m) / (kPiL * m);  // phi(0)
     win.b[t] = (R)b;
     win.inv_den[t] = (R)(1.0L / (-expm1l(-2.0L * b * m)));
+    win.tail[t] = (R)(2.0L * expl(-b * m) / (-expm1l(-2.0L * b * m)));
     for (long ks = 0; ks < N[t]; ks++) pinv.push_back((R)(peak / phi_hut(n[t], ks - N[t] / 2, m, b)));
   }
   NFFT_HIP_CHECK(R, hipMalloc(&phi_inv, sizeof(R) * pinv.size()));
   NFFT_HIP_CHECK(R, hipMemcpy(phi_inv, pinv.data(), sizeof(R) * pinv.size(), hipMemcpyHostToDevice));
+  {
+    std::vector<R> pc;
+    for (int t = 0; t < d; t++) {
+      long double sigma = (long double)n[t] / (long double)N[t];
+      long double b = kPiL * (2.0L - 1.0L / sigma);
+      fit_window_polys<R>(m, b, pc);
+    }
+    NFFT_HIP_CHECK(R, hipMalloc(&poly, sizeof(R) * pc.size()));
+    NFFT_HIP_CHECK(R, hipMemcpy(poly, pc.data(), sizeof(R) * pc.size(), hipMemcpyHostToDevice));
+    win.poly = poly;
+  }
 
   // bins: shrink the largest edge until the LDS tile fits the budget
   const size_t budget = 48 * 1024;
@@ -146,15 +214,17 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   auto variant = [](const char *name, int dflt) {
     const char *v = std::getenv(name);
     if (!v) return dflt;
-    return std::strcmp(v, "priv") == 0   ? 2
+    return std::strcmp(v, "rows") == 0   ? 3
+           : std::strcmp(v, "priv") == 0 ? 2
            : std::strcmp(v, "wave") == 0 ? 1
            : std::strcmp(v, "lane") == 0 ? 0
                                          : dflt;
   };
   interp_variant = variant("NFFT_HIP_INTERP", 0);
-  // float: private-tile wavefronts (plain FP32 read-modify-write); double:
-  // wavefronts sharing a tile with ds_add_f64
-  spread_variant = variant("NFFT_HIP_SPREAD", sizeof(R) == 4 ? 2 : 1);
+  // float: plain FP32 read-modify-write by wavefronts that own disjoint
+  // cells (row-partitioned in 3-D, private tiles in 2-D); double: wavefronts
+  // sharing a tile with ds_add_f64
+  spread_variant = variant("NFFT_HIP_SPREAD", sizeof(R) == 4 ? (d == 3 ? 3 : 2) : 1);
 
   // FFT axis passes
   const int cap = sizeof(R) == 4 ? 4096 : 2048;  // elements per workgroup
@@ -195,6 +265,7 @@ void Engine<R>::destroy() {
   hipFree(f);
   hipFree(xdev);
   hipFree(phi_inv);
+  hipFree(poly);
   hipFree(cell);
   hipFree(frac);
   hipFree(perm);

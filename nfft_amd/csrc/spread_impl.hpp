@@ -488,13 +488,39 @@ __global__ __launch_bounds__(64) void k_spread_wave_priv(const cplx<R> *__restri
   const int wt = lane / W < D ? lane / W : D - 1;
   const int ww = lane - (lane / W) * W;
   const int rowstride = D == 3 ? E1 * E2 : 0;
-  for (int i = 0; i < sp.z; i++) {
-    const long p = (long)sp.y + i;
-    R mine = R(0);
-    if (lane < D * W) mine = kb_psi<R>(frac[wt * gg.M + p] + R(m - ww), R(m), wp.b[wt], wp.inv_den[wt]);
+  // node data arrive 64 nodes at a time, one node per lane (coalesced), the
+  // next chunk loading while the current one is processed; node k of a
+  // chunk is broadcast from lane k
+  int cc[D], nc[D];
+  R cfr[D], nfr[D];
+  cplx<R> cf, nf;
+  auto load_chunk = [&](int first, int (&c)[D], R (&fr)[D], cplx<R> &fv) {
+    const int i = first + lane;
+    const bool ok = i < sp.z;
+    const long p = (long)sp.y + (ok ? i : 0);
+#pragma unroll
+    for (int t = 0; t < D; t++) {
+      c[t] = cell[t * gg.M + p];
+      fr[t] = frac[t * gg.M + p];
+    }
+    fv = fin[perm[p]];
+  };
+  load_chunk(0, cc, cfr, cf);
+  for (int first = 0; first < sp.z; first += 64) {
+    if (first + 64 < sp.z) load_chunk(first + 64, nc, nfr, nf);
+    const int cnt = min(64, sp.z - first);
+    for (int k = 0; k < cnt; k++) {
+    R frk[D];
     int a[D];
 #pragma unroll
-    for (int t = 0; t < D; t++) a[t] = cell[t * gg.M + p] - lo[t];
+    for (int t = 0; t < D; t++) {
+      frk[t] = lane_bcast<R>(cfr[t], k);
+      a[t] = __builtin_amdgcn_readlane(cc[t], k) - lo[t];
+    }
+    const R frw = D == 3 ? (wt == 0 ? frk[0] : wt == 1 ? frk[1] : frk[D - 1])
+                         : (wt == 0 ? frk[0] : frk[D - 1]);
+    R mine = R(0);
+    if (lane < D * W) mine = kb_psi<R>(frw + R(m - ww), R(m), wp.b[wt], wp.inv_den[wt]);
     R wpair[SLOTS];
 #pragma unroll
     for (int s = 0; s < SLOTS; s++) {
@@ -502,7 +528,7 @@ __global__ __launch_bounds__(64) void k_spread_wave_priv(const cplx<R> *__restri
       const R v = lane_fetch<R>(mine, (D - 1) * W + pb[s]);
       wpair[s] = pv[s] ? u * v : R(0);
     }
-    const cplx<R> fj = fin[perm[p]];
+    const cplx<R> fj = cplx<R>{lane_bcast<R>(cf.x, k), lane_bcast<R>(cf.y, k)};
     cplx<R> *base = D == 3 ? tile + ((a[0] * E1 + a[1]) * E2 + a[2]) : tile + (a[0] * E1 + a[1]);
     // load every cell this lane updates for this node, then update, then store
     cplx<R> v[ROWS][SLOTS];
@@ -526,6 +552,13 @@ __global__ __launch_bounds__(64) void k_spread_wave_priv(const cplx<R> *__restri
 #pragma unroll
       for (int s = 0; s < SLOTS; s++)
         if (pv[s]) base[r * rowstride + off[s]] = v[r][s];
+    }
+#pragma unroll
+    for (int t = 0; t < D; t++) {
+      cc[t] = nc[t];
+      cfr[t] = nfr[t];
+    }
+    cf = nf;
   }
   __syncthreads();
   for (int q = lane; q < tot; q += 64) {
@@ -534,6 +567,150 @@ __global__ __launch_bounds__(64) void k_spread_wave_priv(const cplx<R> *__restri
     else if (v.x != R(0) || v.y != R(0)) global_add<R>(g + region_to_grid<D>(q, lo, gg), v.x, v.y);
   }
 }
+
+// ================================================== row-partitioned B^T, d = 3
+// NW wavefronts share one LDS tile and split it by the parity class of the
+// first tile coordinate: wavefront w updates only tile rows e0 with
+// e0 % NW == w, for every node of the work item.  Rows never overlap between
+// wavefronts and pairs never overlap within one instruction, so plain FP32
+// read-modify-write is race-free; each wavefront does 1/NW of the taps and
+// the full per-node setup, and the tile is shared (NW times the occupancy of
+// the private-tile kernel at the same LDS).  Invalid pair slots (lanes past
+// W*W) read and rewrite a dummy cell past the tile instead of branching.
+template <typename R, int D, int W, int NW>
+__global__ __launch_bounds__(64 * NW) void k_spread_rows(const cplx<R> *__restrict__ fin,
+                                                         cplx<R> *__restrict__ g,
+                                                         const int *__restrict__ perm,
+                                                         const int *__restrict__ cell,
+                                                         const R *__restrict__ frac,
+                                                         const int4 *__restrict__ subs,
+                                                         BinGeom gg, WinParams<R> wp,
+                                                         cplx<R> *__restrict__ tiles) {
+  static_assert(D == 3 && W % NW == 0, "row-partitioned spreading covers d = 3");
+  constexpr int m = W / 2 - 1;
+  constexpr int PAIRS = W * W;
+  constexpr int SLOTS = (PAIRS + 63) / 64;
+  constexpr int RW = W / NW;  // rows per wavefront and node
+  extern __shared__ unsigned char b_smem[];
+  cplx<R> *tile = reinterpret_cast<cplx<R> *>(b_smem);
+  const int4 sp = subs[blockIdx.x];
+  int lo[3];
+  bin_corner<3>(sp.x, gg, lo);
+  const int E1 = gg.E[1], E2 = gg.E[2];
+  const int tot = gg.E[0] * E1 * E2;
+  for (int q = threadIdx.x; q <= tot; q += blockDim.x) tile[q] = cplx<R>{R(0), R(0)};
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int rowstride = E1 * E2;
+  // this lane's pairs over dims 1, 2: tile offset within a row; invalid
+  // slots point at the dummy cell (index tot) relative to the row base
+  int pa[SLOTS], pb[SLOTS], off[SLOTS];
+  bool pv[SLOTS];
+#pragma unroll
+  for (int s = 0; s < SLOTS; s++) {
+    const int q = lane + 64 * s;
+    pv[s] = q < PAIRS;
+    pa[s] = pv[s] ? q / W : 0;
+    pb[s] = pv[s] ? q - (q / W) * W : 0;
+    off[s] = pa[s] * E2 + pb[s];
+  }
+  const int wt = lane / W < 3 ? lane / W : 2;
+  const int ww = lane - (lane / W) * W;
+  int cc[3], nc[3];
+  R cfr[3], nfr[3];
+  cplx<R> cf, nf;
+  auto load_chunk = [&](int first, int (&c)[3], R (&fr)[3], cplx<R> &fv) {
+    const int i = first + lane;
+    const long p = (long)sp.y + (i < sp.z ? i : 0);
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      c[t] = cell[t * gg.M + p];
+      fr[t] = frac[t * gg.M + p];
+    }
+    fv = fin[perm[p]];
+  };
+  load_chunk(0, cc, cfr, cf);
+  for (int first = 0; first < sp.z; first += 64) {
+    if (first + 64 < sp.z) load_chunk(first + 64, nc, nfr, nf);
+    const int cnt = min(64, sp.z - first);
+    for (int k = 0; k < cnt; k++) {
+      R frk[3];
+      int a[3];
+#pragma unroll
+      for (int t = 0; t < 3; t++) {
+        frk[t] = lane_bcast<R>(cfr[t], k);
+        a[t] = __builtin_amdgcn_readlane(cc[t], k) - lo[t];
+      }
+      const R frw = wt == 0 ? frk[0] : (wt == 1 ? frk[1] : frk[2]);
+      R mine = R(0);
+      if constexpr (sizeof(R) == 4) {
+        mine = kb_psi_fast(frw + R(m - ww), R(m), wp.b[wt], wp.inv_den[wt], wp.tail[wt]);
+      } else {
+        mine = kb_psi<R>(frw + R(m - ww), R(m), wp.b[wt], wp.inv_den[wt]);
+      }
+      R wpair[SLOTS];
+      int addr[SLOTS];
+      const int row_base = (a[1] * E2 + a[2]);
+#pragma unroll
+      for (int s = 0; s < SLOTS; s++) {
+        const R u = lane_fetch<R>(mine, W + pa[s]);
+        const R v = lane_fetch<R>(mine, 2 * W + pb[s]);
+        wpair[s] = pv[s] ? u * v : R(0);
+        addr[s] = row_base + off[s];
+      }
+      const cplx<R> fj = cplx<R>{lane_bcast<R>(cf.x, k), lane_bcast<R>(cf.y, k)};
+      // first of this wavefront's rows: smallest r >= 0 with (a0 + r) % NW == wv
+      const int r0 = ((wv - a[0]) % NW + NW) % NW;
+      cplx<R> v[RW][SLOTS];
+#pragma unroll
+      for (int j = 0; j < RW; j++) {
+        const int e0 = a[0] + r0 + j * NW;
+#pragma unroll
+        for (int s = 0; s < SLOTS; s++) {
+          const int idx = pv[s] ? e0 * rowstride + addr[s] : tot;
+          v[j][s] = tile[idx];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < RW; j++) {
+        // r0 < NW and j < W / NW, so every row r is a valid tap
+        const R p0 = lane_bcast<R>(mine, r0 + j * NW);
+        const R wr = p0 * fj.x, wi = p0 * fj.y;
+#pragma unroll
+        for (int s = 0; s < SLOTS; s++) {
+          v[j][s].x += wpair[s] * wr;
+          v[j][s].y += wpair[s] * wi;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < RW; j++) {
+        const int e0 = a[0] + r0 + j * NW;
+#pragma unroll
+        for (int s = 0; s < SLOTS; s++) {
+          const int idx = pv[s] ? e0 * rowstride + addr[s] : tot;
+          tile[idx] = v[j][s];
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      cc[t] = nc[t];
+      cfr[t] = nfr[t];
+    }
+    cf = nf;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < tot; q += blockDim.x) {
+    const cplx<R> v = tile[q];
+    if (tiles) tiles[(long)blockIdx.x * tot + q] = v;
+    else if (v.x != R(0) || v.y != R(0)) global_add<R>(g + region_to_grid<3>(q, lo, gg), v.x, v.y);
+  }
+}
+
+}  // namespace nfft_hip
+#include "spread_rows.hpp"
+namespace nfft_hip {
 
 // ==================================================================== runtime m
 // Window values and wrapped tap coordinates live in per-thread arrays.
@@ -824,6 +1001,7 @@ template <typename R> struct BKernelTable {
   fn interp_wave[kMaxDim][kMaxStaticM + 1];  // d = 2, 3 only
   fn spread_wave[kMaxDim][kMaxStaticM + 1];
   fn spread_priv[kMaxDim][kMaxStaticM + 1];  // d = 2, 3 only, 64 threads
+  fn spread_rows[kMaxStaticM + 1];            // d = 3 only, 128 threads
   fn interp_rt[kMaxDim][2];  // [d-1][tile]
   fn spread_rt[kMaxDim][2];
 };
@@ -837,6 +1015,7 @@ void bk_fill_one(BKernelTable<R> &k) {
     k.spread_wave[D - 1][M_] = k_bwave_tile<R, D, 2 * M_ + 2, true>;
     k.spread_priv[D - 1][M_] = k_spread_wave_priv<R, D, 2 * M_ + 2>;
   }
+  if constexpr (D == 3 && D * (2 * M_ + 2) <= 64) k.spread_rows[M_] = k_spread_rows2<R, 2 * M_ + 2, 2>;
 }
 
 template <typename R, int D, int... Ms>
@@ -879,7 +1058,12 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
   threads = 256;
   if (tile && gg.m >= 1 && gg.m <= kMaxStaticM) {
     const int t = gg.d - 1;
-    if (spread && variant == 2 && k.spread_priv[t][gg.m]) {
+    if (spread && variant == 3 && gg.d == 3 && k.spread_rows[gg.m]) {
+      threads = 128;
+      lds = bytes + 2 * sizeof(cplx<R>);  // two dummy cells past the tile
+      return k.spread_rows[gg.m];
+    }
+    if (spread && variant >= 2 && k.spread_priv[t][gg.m]) {
       threads = 64;
       return k.spread_priv[t][gg.m];
     }

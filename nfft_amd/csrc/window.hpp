@@ -31,4 +31,21 @@ __device__ __forceinline__ R kb_psi(R y, R m, R b, R inv_den) {
   return R(2) * b * m * exp(-b * m) * inv_den;
 }
 
+// The same window without branches, for FP32 wavefront kernels where every
+// lane of a wave evaluates one tap.  The peak branch keeps the accurate exp;
+// the factors that only shape the smallest values (1 - exp(-2bs) near |y| = m
+// and the sin branch beyond it, both below 1e-6 of the peak) use the fast
+// hardware exp / sin.  tail = 2 exp(-b m) / (1 - exp(-2 b m)).
+__device__ __forceinline__ float kb_psi_fast(float y, float m, float b, float inv_den,
+                                             float tail) {
+  const float q = m * m - y * y;
+  const float aq = fabsf(q);
+  const float rs = __frsqrt_rn(aq);  // 1 / s
+  const float s = aq * rs;
+  const float pos = (m * rs) * expf(b * (s - m)) * (1.f - __expf(-2.f * b * s)) * inv_den;
+  const float neg = (m * rs) * __sinf(b * s) * tail;
+  const float zero = b * m * tail;
+  return q > 0.f ? pos : (q < 0.f ? neg : zero);
+}
+
 }  // namespace nfft_hip
