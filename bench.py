@@ -89,6 +89,9 @@ def main():
     ap.add_argument("--samples", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many bytes "
+                         "(calibrates the HBM byte counters of a profiler run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -98,6 +101,16 @@ def main():
     import torch
     import nfft_amd as nf
     from nfft_amd import workloads as W
+
+    if args.copy_probe:
+        n = args.copy_probe // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 4, "copies": 3}))
+        return
 
     N = [128, 128, 128]
     n = [256, 256, 256]

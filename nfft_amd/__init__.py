@@ -302,6 +302,80 @@ class Plan:
             pass
 
 
+class Solver:
+    """A solver_plan_complex / solverf_plan_complex on a Plan, driven through
+    the C ABI (solver_init_advanced_complex, solver_before_loop_complex,
+    solver_loop_one_step_complex, solver_finalize_complex; reference
+    include/nfft3.h:782-815).  y, w, w_hat, f_hat_iter, r_iter, ... are
+    numpy views of the solver's host arrays."""
+
+    def __init__(self, plan, flags=CGNR):
+        self.plan = plan
+        self.L = plan.L
+        self.pre = "solver_" if plan.precision == "double" else "solverf_"
+        self.s = (solver_plan_complex if plan.precision == "double" else solverf_plan_complex)()
+        self.sp = ctypes.pointer(self.s)
+        mv = ctypes.cast(plan.pp, ctypes.c_void_p)
+        getattr(self.L, self.pre + "init_advanced_complex")(self.sp, mv, int(flags))
+        self.alive = True
+
+    def _c(self, name, n):
+        return _view(getattr(self.s, name), n, self.plan.cdtype)
+
+    def _r(self, name, n):
+        return _view(getattr(self.s, name), n, self.plan.rdtype)
+
+    @property
+    def y(self):
+        return self._c("y", self.plan.M)
+
+    @property
+    def f_hat_iter(self):
+        return self._c("f_hat_iter", self.plan.p.N_total)
+
+    @property
+    def r_iter(self):
+        return self._c("r_iter", self.plan.M)
+
+    @property
+    def z_hat_iter(self):
+        return self._c("z_hat_iter", self.plan.p.N_total)
+
+    @property
+    def p_hat_iter(self):
+        return self._c("p_hat_iter", self.plan.p.N_total)
+
+    @property
+    def w(self):
+        return self._r("w", self.plan.M)
+
+    @property
+    def w_hat(self):
+        return self._r("w_hat", self.plan.p.N_total)
+
+    def before_loop(self):
+        getattr(self.L, self.pre + "before_loop_complex")(self.sp)
+
+    def loop_one_step(self):
+        getattr(self.L, self.pre + "loop_one_step_complex")(self.sp)
+
+    def set_sync(self, level):
+        fn = getattr(self.L, self.pre + "set_sync_complex")
+        fn.argtypes = [type(self.sp), ctypes.c_int]
+        fn(self.sp, int(level))
+
+    def finalize(self):
+        if self.alive:
+            getattr(self.L, self.pre + "finalize_complex")(self.sp)
+            self.alive = False
+
+    def __del__(self):
+        try:
+            self.finalize()
+        except Exception:
+            pass
+
+
 def _next_pow2(x):
     if x < 2:
         return x + 1

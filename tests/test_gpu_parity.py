@@ -212,3 +212,20 @@ def test_pointer_swap_between_calls():
         assert rel_errs(out, ref)[0] < 1e-12
     finally:
         p.finalize()
+
+
+@pytest.mark.gpu
+def test_c_caller_runs(tmp_path):
+    """tests/c/simple_test.c, a C caller of the plan API written like the
+    reference's examples, compiled against include/nfft3.h and run."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "simple_test"
+    subprocess.run(["gcc", "-std=c99", "-O1", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "c", "simple_test.c"), "-o", str(exe),
+                    "-L", os.path.join(root, "nfft_amd"), "-lnfft3_hip", "-lm",
+                    "-Wl,-rpath," + os.path.join(root, "nfft_amd")], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0 and "OK" in r.stdout, r.stdout + r.stderr

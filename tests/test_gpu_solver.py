@@ -1,0 +1,109 @@
+"""Iterative inverse NFFT through the C ABI (GPU) against the solver oracle.
+
+The oracle (oracle/solver.py, restating kernel/solver/solver.c and pinned
+against least squares on CPU) runs the same recurrences with the FP64 NFFT
+oracle as operator.  The library's device CGNR (plan is one of its own) and
+its host path (the other methods) must follow the same iterates:
+  double family: f_hat_iter and dot_* within 1e-9 relative for 8 steps
+  float family:  within 2e-4 relative (FP32 operator, FP64 inner products)
+"""
+import numpy as np
+import pytest
+
+import nfft_amd as nf
+import oracle as O
+from oracle import solver as S
+from nfft_amd import workloads as W
+
+FLAGS = nf.PRE_PHI_HUT | nf.PRE_PSI | nf.MALLOC_X | nf.MALLOC_F_HAT | nf.MALLOC_F | nf.FFTW_INIT | nf.FFT_OUT_OF_PLACE
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def run_case(method, extra, precision, steps=8, seed=5):
+    N = [24, 20]
+    m = 4
+    x = W.clustered_2d(900, seed=seed)
+    M = x.shape[0]
+    NN = int(np.prod(N))
+    p = nf.Plan(N, M, m=m, n=[48, 40], flags=FLAGS, precision=precision)
+    p.x[:] = x
+    p.precompute()
+    xr = p.x.astype(np.float64).copy()
+    n = p.n
+    r = np.random.default_rng(seed)
+    f_true = W.complex_normal(NN, seed=seed + 1)
+    y = O.nfft_trafo(N, n, m, xr, f_true) + 0.01 * W.complex_normal(M, seed=seed + 2)
+    flags = method | extra
+    s = nf.Solver(p, flags)
+    o = S.Solver(lambda f: O.nfft_trafo(N, n, m, xr, f), lambda v: O.nfft_adjoint(N, n, m, xr, v), M, NN, flags)
+    try:
+        s.y[:] = y
+        o.y = s.y.astype(np.complex128).copy()
+        s.f_hat_iter[:] = 0
+        o.f_hat_iter = np.zeros(NN, complex)
+        if flags & nf.PRECOMPUTE_WEIGHT:
+            w = r.uniform(0.5, 2.0, M)
+            s.w[:] = w
+            o.w = s.w.astype(np.float64).copy()
+        if flags & nf.PRECOMPUTE_DAMP:
+            wh = r.uniform(0.5, 2.0, NN)
+            s.w_hat[:] = wh
+            o.w_hat = s.w_hat.astype(np.float64).copy()
+        if flags & nf.LANDWEBER:
+            s.s.alpha_iter = 0.5 / M
+            o.alpha_iter = float(s.s.alpha_iter)
+        s.before_loop()
+        o.before_loop()
+        out = []
+        for it in range(steps):
+            s.loop_one_step()
+            o.loop_one_step()
+            out.append((rel(s.f_hat_iter.astype(np.complex128), o.f_hat_iter),
+                        abs(s.s.dot_r_iter - o.dot_r_iter) / abs(o.dot_r_iter),
+                        rel(s.r_iter.astype(np.complex128), o.r_iter)))
+        return out
+    finally:
+        s.finalize()
+        p.finalize()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("extra", [0, nf.PRECOMPUTE_WEIGHT, nf.PRECOMPUTE_WEIGHT | nf.PRECOMPUTE_DAMP])
+@pytest.mark.parametrize("method", [nf.CGNR, nf.STEEPEST_DESCENT, nf.CGNE,
+                                    nf.LANDWEBER | nf.NORMS_FOR_LANDWEBER])
+def test_solver_vs_oracle(method, extra, precision):
+    if method == nf.CGNE and extra:
+        pytest.skip("CGNE on an overdetermined noisy system is only run plain")
+    steps = 4 if method == nf.CGNE else 8
+    res = run_case(method, extra, precision, steps=steps)
+    tol = 1e-9 if precision == "double" else 2e-4
+    for it, (ef, er, err) in enumerate(res):
+        assert ef < tol and er < tol and err < tol, (it, ef, er, err)
+
+
+@pytest.mark.gpu
+def test_device_cgnr_sync_levels():
+    """Sync level 1 keeps f_hat_iter and the scalars current on the host."""
+    N = [16, 16]
+    x = W.uniform(500, 2, seed=71)
+    p = nf.Plan(N, 500, m=4, n=[32, 32], flags=FLAGS, precision="double")
+    p.x[:] = x
+    p.precompute()
+    s = nf.Solver(p, nf.CGNR)
+    try:
+        s.set_sync(1)
+        s.y[:] = W.complex_normal(500, seed=72)
+        s.f_hat_iter[:] = 0
+        s.before_loop()
+        d0 = s.s.dot_z_hat_iter
+        for _ in range(5):
+            s.loop_one_step()
+        assert s.s.dot_z_hat_iter < d0
+        assert np.linalg.norm(s.f_hat_iter) > 0
+    finally:
+        s.finalize()
+        p.finalize()
