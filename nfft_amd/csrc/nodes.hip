@@ -7,10 +7,14 @@
 //   c_t = floor(n_t x_t),  frac_t = n_t x_t - c_t in [0,1)
 // The reference's tap window u = c - m .. c + m + 1 (kernel/nfft/nfft.c:324-340)
 // follows from c.  Instead of the reference's radix sort of linearised cell
-// keys, nodes are counting-sorted into bins of bs_t cells per dimension
-// (histogram with atomics, exclusive scan, placement), and every bin is cut
-// into work items of at most max_per_sub nodes for the LDS-tiled B kernels.
-// Order inside a bin is not specified; results do not depend on it.
+// keys, nodes are counting-sorted by a bin-major cell key
+//   key = bin * (bs_0 ... bs_{d-1}) + (cell offset inside the bin, row-major)
+// (histogram with atomics, multi-block exclusive scan, placement), so each bin
+// is one contiguous range and, inside it, nodes of one cell are adjacent (the
+// spreading kernels sum such runs in registers).  Every bin is cut into work
+// items of at most max_per_sub nodes for the LDS-tiled B kernels.  Order
+// inside a cell is not specified; results do not depend on it beyond
+// rounding.
 #include <cstdio>
 
 #include "engine.hpp"
@@ -18,12 +22,13 @@
 namespace nfft_hip {
 
 template <typename R>
-__global__ void k_node_prep(const R *__restrict__ x, BinGeom gg, int *__restrict__ cell_tmp,
-                            R *__restrict__ frac_tmp, int *__restrict__ bin_of,
-                            int *__restrict__ rank_of, int *__restrict__ hist) {
+__global__ void k_node_prep(const R *__restrict__ x, BinGeom gg, int cap,
+                            int *__restrict__ cell_tmp, R *__restrict__ frac_tmp,
+                            int *__restrict__ key_of, int *__restrict__ rank_of,
+                            int *__restrict__ khist) {
   long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= gg.M) return;
-  int bin = 0;
+  int bin = 0, local = 0;
   for (int t = 0; t < gg.d; t++) {
     double y = (double)x[j * gg.d + t] * (double)gg.n[t];
     double c = floor(y);
@@ -32,13 +37,17 @@ __global__ void k_node_prep(const R *__restrict__ x, BinGeom gg, int *__restrict
     if (cw < 0) cw += gg.n[t];
     cell_tmp[t * gg.M + j] = (int)cw;
     frac_tmp[t * gg.M + j] = (R)fr;
-    bin = bin * gg.nb[t] + (int)cw / gg.bs[t];
+    const int bt = (int)cw / gg.bs[t];
+    bin = bin * gg.nb[t] + bt;
+    local = local * gg.bs[t] + ((int)cw - bt * gg.bs[t]);
   }
-  bin_of[j] = bin;
-  rank_of[j] = atomicAdd(&hist[bin], 1);
+  const int key = bin * cap + local;
+  key_of[j] = key;
+  rank_of[j] = atomicAdd(&khist[key], 1);
 }
 
 // Exclusive scan of `len` ints by one workgroup of 1024 threads; total in *sum.
+// Used for short arrays (bins, block sums of the long scan below).
 __global__ __launch_bounds__(1024) void k_scan_exclusive(const int *__restrict__ in,
                                                          int *__restrict__ out, long len,
                                                          int *__restrict__ sum) {
@@ -67,14 +76,80 @@ __global__ __launch_bounds__(1024) void k_scan_exclusive(const int *__restrict__
   if (tid == 1023) *sum = (int)part[1023];
 }
 
+// Long exclusive scan in three launches: per-tile sums (k_scan_tile_sums),
+// a one-workgroup scan of the tile sums, and per-tile scans with the tile's
+// offset (k_scan_tile_apply).  A tile is 256 threads x 16 consecutive ints.
+constexpr int kScanPerThread = 16;
+constexpr int kScanTile = 256 * kScanPerThread;
+
+__device__ inline int block_exclusive_sum_256(int v, int *sh, int &total) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int u = tid >= off ? sh[tid - off] : 0;
+    __syncthreads();
+    sh[tid] += u;
+    __syncthreads();
+  }
+  total = sh[255];
+  const int incl = sh[tid];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tile_sums(const int *__restrict__ in, long len,
+                                                        int *__restrict__ tile_sum) {
+  __shared__ int sh[256];
+  const long q0 = (long)blockIdx.x * kScanTile + threadIdx.x * kScanPerThread;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPerThread; k++) s += q0 + k < len ? in[q0 + k] : 0;
+  int total;
+  block_exclusive_sum_256(s, sh, total);
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tile_apply(const int *__restrict__ in,
+                                                         int *__restrict__ out, long len,
+                                                         const int *__restrict__ tile_off) {
+  __shared__ int sh[256];
+  const long q0 = (long)blockIdx.x * kScanTile + threadIdx.x * kScanPerThread;
+  int v[kScanPerThread];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPerThread; k++) {
+    v[k] = q0 + k < len ? in[q0 + k] : 0;
+    s += v[k];
+  }
+  int total;
+  int run = tile_off[blockIdx.x] + block_exclusive_sum_256(s, sh, total);
+#pragma unroll
+  for (int k = 0; k < kScanPerThread; k++) {
+    if (q0 + k < len) out[q0 + k] = run;
+    run += v[k];
+  }
+}
+
+// bins from the scanned cell-key histogram: bin b owns keys [b cap, (b+1) cap)
+__global__ void k_bins_from_keys(const int *__restrict__ kstart, long nbins, int cap, int total,
+                                 int *__restrict__ hist, int *__restrict__ bin_start) {
+  long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  const int first = kstart[b * cap];
+  const int next = b + 1 < nbins ? kstart[(b + 1) * cap] : total;
+  bin_start[b] = first;
+  hist[b] = next - first;
+}
+
 template <typename R>
 __global__ void k_node_place(BinGeom gg, const int *__restrict__ cell_tmp,
-                             const R *__restrict__ frac_tmp, const int *__restrict__ bin_of,
-                             const int *__restrict__ rank_of, const int *__restrict__ bin_start,
+                             const R *__restrict__ frac_tmp, const int *__restrict__ key_of,
+                             const int *__restrict__ rank_of, const int *__restrict__ kstart,
                              int *__restrict__ cell, R *__restrict__ frac, int *__restrict__ perm) {
   long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= gg.M) return;
-  long p = (long)bin_start[bin_of[j]] + rank_of[j];
+  long p = (long)kstart[key_of[j]] + rank_of[j];
   perm[p] = (int)j;
   for (int t = 0; t < gg.d; t++) {
     cell[t * gg.M + p] = cell_tmp[t * gg.M + j];
@@ -103,46 +178,84 @@ __global__ void k_fill_subs(const int *__restrict__ hist, const int *__restrict_
   }
 }
 
+__global__ void k_fill_zero_int(int *__restrict__ p, long count) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0;
+}
+
+static void launch_fill_zero_int(int *p, long count, hipStream_t s) {
+  if (count > 0)
+    hipLaunchKernelGGL(k_fill_zero_int, dim3((unsigned)ceil_div(count, 256)), dim3(256), 0, s, p,
+                       count);
+}
+
 template <typename R>
 void bin_nodes(Engine<R> &e) {
   const BinGeom &gg = e.geom;
   const long M = e.M;
   long nbins = 1;
-  for (int t = 0; t < gg.d; t++) nbins *= gg.nb[t];
-  // scratch layout, each array 256-byte aligned: hist, bin_start, nsub_of,
-  // sub_start (nbins ints each), bin_of, rank_of (M ints each), cell_tmp
-  // (d M ints), frac_tmp (d M reals), sums (2 ints)
+  int cells_per_bin = 1;
+  for (int t = 0; t < gg.d; t++) {
+    nbins *= gg.nb[t];
+    cells_per_bin *= gg.bs[t];
+  }
+  const long nkeys = nbins * cells_per_bin;
+  if (nkeys > 0x7fffffffL)
+    family<R>::fatal("nfft: %ld grid cells exceed the node binning's int32 keys", nkeys);
+  const long ntiles = ceil_div(nkeys, kScanTile);
+  // scratch layout, each array 256-byte aligned: khist, kstart (nkeys ints),
+  // tile_sum, tile_off (ntiles ints), hist, bin_start, nsub_of, sub_start
+  // (nbins ints each), key_of, rank_of (M ints each), cell_tmp (d M ints),
+  // frac_tmp (d M reals), sums (4 ints)
   auto al = [](long bytes) { return (bytes + 255) / 256 * 256; };
+  const long b_keys = al(nkeys * (long)sizeof(int));
+  const long b_tiles = al(ntiles * (long)sizeof(int));
   const long b_bins = al(nbins * (long)sizeof(int));
   const long b_M = al(M * (long)sizeof(int));
   const long b_cell = al((long)gg.d * M * (long)sizeof(int));
   const long b_frac = al((long)gg.d * M * (long)sizeof(R));
-  const long need = 4 * b_bins + 2 * b_M + b_cell + b_frac + 256;
+  const long need = 2 * b_keys + 2 * b_tiles + 4 * b_bins + 2 * b_M + b_cell + b_frac + 256;
   if (need > e.scratch_cap) {
     if (e.scratch_i) NFFT_HIP_CHECK(R, hipFree(e.scratch_i));
     NFFT_HIP_CHECK(R, hipMalloc(&e.scratch_i, need));
     e.scratch_cap = need;
   }
-  char *base = reinterpret_cast<char *>(e.scratch_i);
-  int *hist = reinterpret_cast<int *>(base);
-  int *bin_start = reinterpret_cast<int *>(base + b_bins);
-  int *nsub_of = reinterpret_cast<int *>(base + 2 * b_bins);
-  int *sub_start = reinterpret_cast<int *>(base + 3 * b_bins);
-  int *bin_of = reinterpret_cast<int *>(base + 4 * b_bins);
-  int *rank_of = reinterpret_cast<int *>(base + 4 * b_bins + b_M);
-  int *cell_tmp = reinterpret_cast<int *>(base + 4 * b_bins + 2 * b_M);
-  R *frac_tmp = reinterpret_cast<R *>(base + 4 * b_bins + 2 * b_M + b_cell);
-  int *sums = reinterpret_cast<int *>(base + 4 * b_bins + 2 * b_M + b_cell + b_frac);
+  char *p = reinterpret_cast<char *>(e.scratch_i);
+  auto take = [&](long bytes) {
+    char *q = p;
+    p += bytes;
+    return q;
+  };
+  int *khist = reinterpret_cast<int *>(take(b_keys));
+  int *kstart = reinterpret_cast<int *>(take(b_keys));
+  int *tile_sum = reinterpret_cast<int *>(take(b_tiles));
+  int *tile_off = reinterpret_cast<int *>(take(b_tiles));
+  int *hist = reinterpret_cast<int *>(take(b_bins));
+  int *bin_start = reinterpret_cast<int *>(take(b_bins));
+  int *nsub_of = reinterpret_cast<int *>(take(b_bins));
+  int *sub_start = reinterpret_cast<int *>(take(b_bins));
+  int *key_of = reinterpret_cast<int *>(take(b_M));
+  int *rank_of = reinterpret_cast<int *>(take(b_M));
+  int *cell_tmp = reinterpret_cast<int *>(take(b_cell));
+  R *frac_tmp = reinterpret_cast<R *>(take(b_frac));
+  int *sums = reinterpret_cast<int *>(take(256));
 
   hipStream_t s = e.stream;
-  NFFT_HIP_CHECK(R, hipMemsetAsync(hist, 0, nbins * sizeof(int), s));
+  launch_fill_zero_int(khist, nkeys, s);
   if (M > 0)
     hipLaunchKernelGGL(k_node_prep<R>, dim3((unsigned)ceil_div(M, 256)), dim3(256), 0, s, e.xdev,
-                       gg, cell_tmp, frac_tmp, bin_of, rank_of, hist);
-  hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, hist, bin_start, nbins, sums);
+                       gg, cells_per_bin, cell_tmp, frac_tmp, key_of, rank_of, khist);
+  hipLaunchKernelGGL(k_scan_tile_sums, dim3((unsigned)ntiles), dim3(256), 0, s, khist, nkeys,
+                     tile_sum);
+  hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, tile_sum, tile_off, ntiles,
+                     sums);
+  hipLaunchKernelGGL(k_scan_tile_apply, dim3((unsigned)ntiles), dim3(256), 0, s, khist, kstart,
+                     nkeys, tile_off);
+  hipLaunchKernelGGL(k_bins_from_keys, dim3((unsigned)ceil_div(nbins, 256)), dim3(256), 0, s,
+                     kstart, nbins, cells_per_bin, (int)M, hist, bin_start);
   if (M > 0)
     hipLaunchKernelGGL(k_node_place<R>, dim3((unsigned)ceil_div(M, 256)), dim3(256), 0, s, gg,
-                       cell_tmp, frac_tmp, bin_of, rank_of, bin_start, e.cell, e.frac, e.perm);
+                       cell_tmp, frac_tmp, key_of, rank_of, kstart, e.cell, e.frac, e.perm);
   // work items of interpolation (small, for balance) and of spreading
   // (larger, to bound the number of tiles written to scratch); the spreading
   // list keeps its per-bin first item and count for the tile reduction

@@ -198,7 +198,7 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     geom.nb[t] = t < d ? (int)ceil_div(n[t], bs[t]) : 1;
     geom.E[t] = t < d ? bs[t] + 2 * m + 1 : 1;
   }
-  max_per_sub = d == 3 ? 256 : 512;
+  max_per_sub = d == 3 ? 1024 : 512;
   spread_max_per_sub = d == 3 ? 2048 : 4096;
   // B^T writes its LDS tiles to scratch and reduces them without atomics
   // when the tile fits LDS and never wraps onto itself (E_t <= n_t)
@@ -214,7 +214,8 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   auto variant = [](const char *name, int dflt) {
     const char *v = std::getenv(name);
     if (!v) return dflt;
-    return std::strcmp(v, "rows") == 0   ? 3
+    return std::strcmp(v, "runs") == 0   ? 4
+           : std::strcmp(v, "rows") == 0 ? 3
            : std::strcmp(v, "priv") == 0 ? 2
            : std::strcmp(v, "wave") == 0 ? 1
            : std::strcmp(v, "lane") == 0 ? 0
@@ -222,9 +223,9 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   };
   interp_variant = variant("NFFT_HIP_INTERP", 0);
   // float: plain FP32 read-modify-write by wavefronts that own disjoint
-  // cells (row-partitioned in 3-D, private tiles in 2-D); double: wavefronts
-  // sharing a tile with ds_add_f64
-  spread_variant = variant("NFFT_HIP_SPREAD", sizeof(R) == 4 ? (d == 3 ? 3 : 2) : 1);
+  // cells (row-partitioned with same-cell runs summed in registers in 3-D,
+  // private tiles in 2-D); double: wavefronts sharing a tile with ds_add_f64
+  spread_variant = variant("NFFT_HIP_SPREAD", sizeof(R) == 4 ? (d == 3 ? 4 : 2) : 1);
 
   // FFT axis passes
   const int cap = sizeof(R) == 4 ? 4096 : 2048;  // elements per workgroup

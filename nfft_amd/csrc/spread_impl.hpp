@@ -22,6 +22,8 @@
 // Algorithmic HBM bytes per launch: n_total c + M d r + M c (B and B^T alike).
 #pragma once
 
+#include <type_traits>
+
 #include "engine.hpp"
 #include "window.hpp"
 
@@ -140,9 +142,58 @@ __device__ __forceinline__ void node_window(const int *__restrict__ cell,
   for (int t = 0; t < D; t++) {
     c[t] = cell[t * M + p];
     const R fr = frac[t * M + p];
+    if constexpr (sizeof(R) == 4) {
+      // per-tap polynomials in (frac - 1/2); coefficients are uniform over
+      // the wavefront and come from scalar loads
+      const R tt = fr - R(0.5);
 #pragma unroll
-    for (int w = 0; w < W; w++)
-      psi[t][w] = kb_psi<R>(fr + R(m - w), R(m), wp.b[t], wp.inv_den[t]);
+      for (int w = 0; w < W; w++) {
+        const R *pc = wp.poly + (t * W + w) * kPolyCoef;
+        R v = pc[kPolyDeg];
+#pragma unroll
+        for (int k = kPolyDeg - 1; k >= 0; k--) v = fmaf(v, tt, pc[k]);
+        psi[t][w] = v;
+      }
+    } else {
+#pragma unroll
+      for (int w = 0; w < W; w++)
+        psi[t][w] = kb_psi<R>(fr + R(m - w), R(m), wp.b[t], wp.inv_den[t]);
+    }
+  }
+}
+
+// Stages the grid region of a 3-D work item (E0 x E1 x E2 cells from
+// lo - m, periodic) into LDS.  Each thread keeps one position along the last
+// dimension and walks whole rows, so per element it does one table lookup
+// per leading dimension instead of divisions and modulos.
+template <typename R>
+__device__ __forceinline__ void load_region_3d(const cplx<R> *__restrict__ g, cplx<R> *tile,
+                                               const int (&lo)[3], const BinGeom &gg,
+                                               long *rowoff0, long *rowoff1) {
+  const int E0 = gg.E[0], E1 = gg.E[1], E2 = gg.E[2];
+  auto wrap = [](int v, int n) {
+    v %= n;
+    return v < 0 ? v + n : v;
+  };
+  for (int e = threadIdx.x; e < E0; e += blockDim.x)
+    rowoff0[e] = (long)wrap(lo[0] - gg.m + e, gg.n[0]) * gg.n[1] * gg.n[2];
+  for (int e = threadIdx.x; e < E1; e += blockDim.x)
+    rowoff1[e] = (long)wrap(lo[1] - gg.m + e, gg.n[1]) * gg.n[2];
+  __syncthreads();
+  const int rows_per_pass = blockDim.x / E2;
+  const int e2 = threadIdx.x % E2;
+  int r = threadIdx.x / E2;
+  if (r < rows_per_pass) {
+    const long col = wrap(lo[2] - gg.m + e2, gg.n[2]);
+    int e0 = r / E1, e1 = r - (r / E1) * E1;
+    for (; e0 < E0;) {
+      tile[(e0 * E1 + e1) * E2 + e2] = g[rowoff0[e0] + rowoff1[e1] + col];
+      e1 += rows_per_pass;
+      while (e1 >= E1) {
+        e1 -= E1;
+        e0++;
+      }
+    }
   }
 }
 
@@ -162,7 +213,12 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
   int tot = 1;
 #pragma unroll
   for (int t = 0; t < D; t++) tot *= gg.E[t];
-  for (int q = threadIdx.x; q < tot; q += blockDim.x) tile[q] = g[region_to_grid<D>(q, lo, gg)];
+  if constexpr (D == 3) {
+    __shared__ long rowoff[2][64];
+    load_region_3d<R>(g, tile, lo, gg, rowoff[0], rowoff[1]);
+  } else {
+    for (int q = threadIdx.x; q < tot; q += blockDim.x) tile[q] = g[region_to_grid<D>(q, lo, gg)];
+  }
   const int E1 = D >= 2 ? gg.E[1] : 1;
   const int E2 = D == 3 ? gg.E[2] : 1;
   __syncthreads();
@@ -710,6 +766,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_rows(const cplx<R> *__restri
 
 }  // namespace nfft_hip
 #include "spread_rows.hpp"
+#include "spread_runs.hpp"
 namespace nfft_hip {
 
 // ==================================================================== runtime m
@@ -1002,6 +1059,7 @@ template <typename R> struct BKernelTable {
   fn spread_wave[kMaxDim][kMaxStaticM + 1];
   fn spread_priv[kMaxDim][kMaxStaticM + 1];  // d = 2, 3 only, 64 threads
   fn spread_rows[kMaxStaticM + 1];            // d = 3 only, 128 threads
+  fn spread_runs[kMaxStaticM + 1];            // d = 3, float only, 128 threads
   fn interp_rt[kMaxDim][2];  // [d-1][tile]
   fn spread_rt[kMaxDim][2];
 };
@@ -1016,6 +1074,8 @@ void bk_fill_one(BKernelTable<R> &k) {
     k.spread_priv[D - 1][M_] = k_spread_wave_priv<R, D, 2 * M_ + 2>;
   }
   if constexpr (D == 3 && D * (2 * M_ + 2) <= 64) k.spread_rows[M_] = k_spread_rows2<R, 2 * M_ + 2, 2>;
+  if constexpr (std::is_same_v<R, float> && D == 3 && D * (2 * M_ + 2) <= 64)
+    k.spread_runs[M_] = k_spread_runs<2 * M_ + 2, 2, 8>;
 }
 
 template <typename R, int D, int... Ms>
@@ -1058,7 +1118,11 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
   threads = 256;
   if (tile && gg.m >= 1 && gg.m <= kMaxStaticM) {
     const int t = gg.d - 1;
-    if (spread && variant == 3 && gg.d == 3 && k.spread_rows[gg.m]) {
+    if (spread && variant == 4 && gg.d == 3 && k.spread_runs[gg.m]) {
+      threads = 128;
+      return k.spread_runs[gg.m];
+    }
+    if (spread && variant >= 3 && gg.d == 3 && k.spread_rows[gg.m]) {
       threads = 128;
       lds = bytes + 2 * sizeof(cplx<R>);  // two dummy cells past the tile
       return k.spread_rows[gg.m];

@@ -1,0 +1,247 @@
+// B^T for d = 3 in FP32 that sums runs of nodes sharing one grid cell in
+// registers before touching the LDS tile.
+//
+// Nodes of a work item arrive sorted by cell (the binning sorts by a
+// bin-major cell key), so nodes with the same tap window are adjacent.  For a
+// run of such nodes every tap of the window receives sum_j f_j psi_j(tap);
+// each lane keeps its taps' partial sums in registers and the tile sees one
+// read-modify-write per run instead of one per node.  Dense sampling (the
+// centre of radial trajectories) puts several nodes in most occupied cells.
+//
+// Tile ownership as in the row-partitioned kernel: NW wavefronts share one
+// LDS tile and wavefront w updates only tile rows e0 with e0 % NW == w, so
+// plain FP32 read-modify-write is race-free (no ds_add_f32, which is slow on
+// this GPU).  A lane owns one tap of dimension 1 and a pair of adjacent taps
+// of dimension 2; lanes past the last pair are masked off.
+//
+// Window values: G nodes at a time, each lane evaluates one tap (dimension
+// wt, tap ww) of one node as a per-tap polynomial (two nodes per pass when
+// 3 W <= 32), with the node's fractional coordinates and value fetched by
+// ds_bpermute from the lane that loaded the node.  A lane's pair weights
+// psi1 * psi2 are fetched the same way; the row weights psi0 * f_j are
+// broadcast with readlane.
+#pragma once
+
+#include "engine.hpp"
+#include "window.hpp"
+
+namespace nfft_hip {
+
+__device__ __forceinline__ float bperm_f(int byte_addr, float v) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(byte_addr, __float_as_int(v)));
+}
+
+__device__ __forceinline__ float rlane_f(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <int W, int NW, int G>
+__global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restrict__ fin,
+                                                         float2 *__restrict__ g,
+                                                         const int *__restrict__ perm,
+                                                         const int *__restrict__ cell,
+                                                         const float *__restrict__ frac,
+                                                         const int4 *__restrict__ subs,
+                                                         BinGeom gg, WinParams<float> wp,
+                                                         float2 *__restrict__ tiles) {
+  static_assert(W % 2 == 0 && W % NW == 0 && 3 * W <= 64, "W = 2m+2 taps, rows split evenly");
+  constexpr int HW = W / 2;
+  constexpr int PAIRS = W * HW;
+  constexpr int SLOTS = (PAIRS + 63) / 64;
+  constexpr int RW = W / NW;
+  constexpr int NP = 3 * W <= 32 ? 2 : 1;  // nodes per window pass
+  constexpr int NPASS = G / NP;
+  static_assert(G % NP == 0 && 64 % G == 0, "groups tile the 64-node chunk");
+  extern __shared__ unsigned char b_smem[];
+  float2 *tile = reinterpret_cast<float2 *>(b_smem);
+  const int4 sp = subs[blockIdx.x];
+  int lo[3];
+  bin_corner<3>(sp.x, gg, lo);
+  const int E1 = gg.E[1], E2 = gg.E[2];
+  const int plane = E1 * E2;
+  const int tot = gg.E[0] * plane;
+  for (int q = threadIdx.x; q < tot; q += blockDim.x) tile[q] = float2{0.f, 0.f};
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+
+  // (dimension-1 tap, dimension-2 pair) slots of this lane
+  int off[SLOTS];
+  bool pv[SLOTS];
+  int adr_u[NP][SLOTS], adr_v0[NP][SLOTS], adr_v1[NP][SLOTS];
+#pragma unroll
+  for (int s = 0; s < SLOTS; s++) {
+    const int q = lane + 64 * s;
+    pv[s] = q < PAIRS;
+    const int pa = pv[s] ? q / HW : 0;
+    const int pb = pv[s] ? 2 * (q - (q / HW) * HW) : 0;
+    off[s] = pa * E2 + pb;
+#pragma unroll
+    for (int h = 0; h < NP; h++) {
+      adr_u[h][s] = 4 * (h * 32 + W + pa);
+      adr_v0[h][s] = 4 * (h * 32 + 2 * W + pb);
+      adr_v1[h][s] = 4 * (h * 32 + 2 * W + pb + 1);
+    }
+  }
+  // window tap evaluated by this lane in a pass: node half h, dimension wt, tap ww
+  const int h = NP == 2 ? lane >> 5 : 0;
+  const int li = NP == 2 ? lane & 31 : lane;
+  const int wt = li / W < 3 ? li / W : 2;
+  const int ww = li - (li / W) * W;
+  float coef[kPolyCoef];
+  {
+    const float *pc = wp.poly + (wt * W + ww) * kPolyCoef;
+#pragma unroll
+    for (int k = 0; k < kPolyCoef; k++) coef[k] = pc[k];
+  }
+
+  // run accumulators: [row j][slot][pair cell], complex
+  float2 acc[RW][SLOTS][2];
+#pragma unroll
+  for (int j = 0; j < RW; j++)
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) acc[j][s][0] = acc[j][s][1] = float2{0.f, 0.f};
+  // current run: packed tile corner | (row a0 << 20); -1 when none
+  int cur = -1;
+  auto flush = [&]() {
+    const int corner = cur & 0xfffff;
+    const int a0 = cur >> 20;
+    const int r0 = ((wv - a0) % NW + NW) % NW;
+    // all reads, then all writes: the rows are distinct cells
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+      if (pv[s]) {
+        float2 v[RW][2];
+#pragma unroll
+        for (int j = 0; j < RW; j++) {
+          const float2 *t = tile + corner + (r0 + j * NW) * plane + off[s];
+          v[j][0] = t[0];
+          v[j][1] = t[1];
+        }
+#pragma unroll
+        for (int j = 0; j < RW; j++) {
+          float2 *t = tile + corner + (r0 + j * NW) * plane + off[s];
+          t[0] = float2{v[j][0].x + acc[j][s][0].x, v[j][0].y + acc[j][s][0].y};
+          t[1] = float2{v[j][1].x + acc[j][s][1].x, v[j][1].y + acc[j][s][1].y};
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < RW; j++) acc[j][s][0] = acc[j][s][1] = float2{0.f, 0.f};
+    }
+  };
+
+  // chunk of 64 nodes, one per lane: packed corner, frac - 1/2, value.
+  // Loads run two chunks ahead for the node index (perm) and one chunk ahead
+  // for everything else, and are converted only when the chunk becomes
+  // current, so no wait for memory falls inside a chunk's work.
+  auto pos = [&](int first) { return (long)sp.y + (first + lane < sp.z ? first + lane : 0); };
+  int ckey;
+  float cfr[3];
+  float2 cf;
+  int nc[3];
+  float nfr[3];
+  float2 nf;
+  int nperm, nnperm = 0;
+  auto load_raw = [&](int first, int pidx) {
+    const long p = pos(first);
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      nc[t] = cell[t * gg.M + p];
+      nfr[t] = frac[t * gg.M + p];
+    }
+    nf = fin[pidx];
+  };
+  auto convert = [&]() {
+    int a[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      a[t] = nc[t] - lo[t];
+      cfr[t] = nfr[t] - 0.5f;
+    }
+    ckey = ((a[0] * E1 + a[1]) * E2 + a[2]) | (a[0] << 20);
+    cf = nf;
+  };
+  load_raw(0, perm[pos(0)]);
+  nperm = perm[pos(64)];
+  convert();
+  for (int first = 0; first < sp.z; first += 64) {
+    if (first + 64 < sp.z) {
+      load_raw(first + 64, nperm);
+      nnperm = perm[pos(first + 128)];
+    }
+    const int cnt = min(64, sp.z - first);
+    for (int base = 0; base < cnt; base += G) {
+      // window values of nodes base .. base+G-1 (lanes past cnt: unused)
+      float mine[NPASS], xr[NPASS], xi[NPASS];
+#pragma unroll
+      for (int p = 0; p < NPASS; p++) {
+        const int src = 4 * (base + NP * p + h);
+        const float f0 = bperm_f(src, cfr[0]);
+        const float f1 = bperm_f(src, cfr[1]);
+        const float f2 = bperm_f(src, cfr[2]);
+        const float fx = bperm_f(src, cf.x);
+        const float fy = bperm_f(src, cf.y);
+        const float tt = wt == 0 ? f0 : (wt == 1 ? f1 : f2);
+        float v = coef[kPolyDeg];
+#pragma unroll
+        for (int k = kPolyDeg - 1; k >= 0; k--) v = fmaf(v, tt, coef[k]);
+        mine[p] = v;
+        xr[p] = v * fx;
+        xi[p] = v * fy;
+      }
+      float w0[G][SLOTS], w1[G][SLOTS];
+#pragma unroll
+      for (int k = 0; k < G; k++) {
+        const int p = k / NP, hk = k % NP;
+#pragma unroll
+        for (int s = 0; s < SLOTS; s++) {
+          const float u = bperm_f(adr_u[hk][s], mine[p]);
+          w0[k][s] = u * bperm_f(adr_v0[hk][s], mine[p]);
+          w1[k][s] = u * bperm_f(adr_v1[hk][s], mine[p]);
+        }
+      }
+      const int ng = min(G, cnt - base);
+#pragma unroll
+      for (int k = 0; k < G; k++) {
+        if (k < ng) {
+          const int p = k / NP, hk = k % NP;
+          const int key = __builtin_amdgcn_readlane(ckey, base + k);
+          if (key != cur) {
+            if (cur >= 0) flush();
+            cur = key;
+          }
+          const int r0 = ((wv - (key >> 20)) % NW + NW) % NW;
+#pragma unroll
+          for (int j = 0; j < RW; j++) {
+            const int src = hk * 32 + r0 + j * NW;
+            const float sr = rlane_f(xr[p], src);
+            const float si = rlane_f(xi[p], src);
+#pragma unroll
+            for (int s = 0; s < SLOTS; s++) {
+              acc[j][s][0].x = fmaf(w0[k][s], sr, acc[j][s][0].x);
+              acc[j][s][0].y = fmaf(w0[k][s], si, acc[j][s][0].y);
+              acc[j][s][1].x = fmaf(w1[k][s], sr, acc[j][s][1].x);
+              acc[j][s][1].y = fmaf(w1[k][s], si, acc[j][s][1].y);
+            }
+          }
+        }
+      }
+    }
+    if (first + 64 < sp.z) {
+      convert();
+      nperm = nnperm;
+    }
+  }
+  if (cur >= 0) flush();
+  __syncthreads();
+  for (int q = threadIdx.x; q < tot; q += blockDim.x) {
+    const float2 v = tile[q];
+    if (tiles) tiles[(long)blockIdx.x * tot + q] = v;
+    else if (v.x != 0.f || v.y != 0.f) {
+      unsafeAtomicAdd(&g[region_to_grid<3>(q, lo, gg)].x, v.x);
+      unsafeAtomicAdd(&g[region_to_grid<3>(q, lo, gg)].y, v.y);
+    }
+  }
+}
+
+}  // namespace nfft_hip
