@@ -66,6 +66,48 @@ NFFT_HIP_DECLARE_EXT(nfft_, nfft_plan)
 NFFT_HIP_DECLARE_SOLVER_EXT(solverf_, solverf_plan_complex)
 NFFT_HIP_DECLARE_SOLVER_EXT(solver_, solver_plan_complex)
 
+/* Density-compensation weights w (M_total floats) for the weighted CGNR
+ * (solverf_init_advanced_complex with PRECOMPUTE_WEIGHT).  No counterpart in
+ * the reference, which reads such weights from files.  Float family only.
+ *
+ * Pipe-Menon fixed point w <- w / (B B^T w) with the plan's own Kaiser-Bessel
+ * interpolation B and spreading B^T, `iters` times from the w given (the
+ * usual start is w = 1).  The _device form takes a device array and is
+ * enqueued on the plan's stream; both return after the work is done. */
+void nfftf_density_pipe_menon(nfftf_plan *ths, float *w, int iters);
+void nfftf_density_pipe_menon_device(nfftf_plan *ths, float *w_dev, int iters);
+
+/* Learned weights: a 3-layer MLP (80 -> 64 -> 64 -> 1, ReLU) on per-node
+ * density features (d = 2 plans): the radius 2|x| and, on coarse grids of
+ * 16, 32 and 64 cells per dimension, log(1 + count / mean count) of the node
+ * counts in the 5 x 5 cells around the node (76 values, zero-padded to 80,
+ * row-major [M_total][80] floats).  The model predicts log w; its dense
+ * layers run on BF16 MFMA with FP32 accumulation.  Parameters are a flat
+ * float array of nfft_dcf_num_params() values: W1 [80][64], b1 [64],
+ * W2 [64][64], b2 [64], w3 [64], b3 (weights in [input][output] order). */
+typedef struct nfft_dcf_model_s nfft_dcf_model;
+int nfft_dcf_num_features(void);
+int nfft_dcf_num_params(void);
+void nfftf_dcf_features_device(nfftf_plan *ths, float *feat_dev);
+nfft_dcf_model *nfft_dcf_model_create(unsigned seed);
+void nfft_dcf_model_destroy(nfft_dcf_model *model);
+void nfft_dcf_model_get_params(const nfft_dcf_model *model, float *params);
+void nfft_dcf_model_set_params(nfft_dcf_model *model, const float *params);
+/* Mean squared error of the predicted log w against target_dev[rows] and its
+ * gradient with respect to every parameter (grad: nfft_dcf_num_params()
+ * floats on the host, may be NULL); the parameters are not changed. */
+float nfft_dcf_model_loss_grad(nfft_dcf_model *model, const float *feat_dev,
+                               const float *target_dev, int rows, float *grad);
+/* One Adam step (beta 0.9 / 0.999) on that loss; returns the loss before
+ * the step. */
+float nfft_dcf_model_train_step(nfft_dcf_model *model, const float *feat_dev,
+                                const float *target_dev, int rows, float lr);
+/* out_dev[rows] = predicted log w */
+void nfft_dcf_model_predict(nfft_dcf_model *model, const float *feat_dev, int rows,
+                            float *out_dev);
+/* w_dev = exp(prediction) on the plan's nodes */
+void nfftf_density_learned_device(nfftf_plan *ths, nfft_dcf_model *model, float *w_dev);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -47,16 +47,43 @@ template <typename R> struct WinParams {
 constexpr int kPolyDeg = 10;
 constexpr int kPolyCoef = kPolyDeg + 1;
 
-// One axis pass of the oversampled FFT: the grid viewed as
-// [outer][len][inner]; each line (outer o, inner i) is transformed.
+// Twiddles and radix schedule of the FFT along one grid dimension.
 template <typename R> struct FftAxis {
   int len;
-  long inner;
+  int nstages;
+  int radix[20];
+  cplx<R> *twiddle;  // exp(-2 pi i k / len), k < len
+};
+
+// Where an FFT pass reads its lines from and writes them to.  The band of
+// dimension t is the N_t grid positions (k - N_t/2) mod n_t, k < N_t, that
+// D fills and D^T reads.
+enum FftIn { kFromGrid = 0, kFromGridBand = 1, kFromCoeffs = 2 };
+enum FftOut { kToGrid = 0, kToGridBand = 1, kToCoeffs = 2 };
+
+// One pruned pass of the oversampled FFT along dimension t (fft.hip): the
+// lines are all combinations of the other dimensions' coordinates, each
+// either over the full grid or over the band only (dimensions whose band is
+// all that is nonzero, or all that is needed).
+template <typename R> struct FftPass {
+  int len;        // n_t
+  long stride;    // element stride of dimension t in the grid
+  int Nband;      // N_t
+  int in_mode, out_mode;
+  int nod;        // other dimensions (d - 1), outer first
+  int cnt[2];     // lines along each other dimension
+  int band[2];    // 1: that coordinate runs over the band
+  int N_od[2], n_od[2];
+  long gstr[2];   // grid strides of the other dimensions
+  long fstr[2];   // f_hat strides of the other dimensions
+  long fstr_ax;   // f_hat stride of dimension t
+  const R *phi_ax;
+  const R *phi_od[2];
   long nlines;
   int lines_per_block;
   int nstages;
   int radix[20];
-  cplx<R> *twiddle;  // exp(-2 pi i k / len), k < len
+  const cplx<R> *twiddle;
 };
 
 template <typename R> struct Engine {
@@ -112,12 +139,18 @@ template <typename R> struct Engine {
   int interp_variant = 0;
   int spread_variant = 1;
   std::vector<FftAxis<R>> axes;
+  std::vector<FftPass<R>> fwd_passes;  // trafo: D fused into the first pass
+  std::vector<FftPass<R>> bwd_passes;  // adjoint: D^T fused into the last pass
 
   void create(int d_, const long *N_, const long *n_, long M_, int m_);
   void destroy();
   void upload_nodes(const R *x_host);
   void build_nodes();  // from xdev
   void trafo(const cplx<R> *fhat_dev, cplx<R> *f_dev);
+  // B^T alone: g <- B^T f (spreading, then the tile reduction or a zeroed
+  // grid with atomics); B alone: f <- B g.  Node data must be built.
+  void spread_to_grid(const cplx<R> *f_dev);
+  void interp_from_grid(cplx<R> *f_dev);
   void adjoint(const cplx<R> *f_dev, cplx<R> *fhat_dev);
   void trafo_direct(const cplx<R> *fhat_dev, cplx<R> *f_dev);
   void adjoint_direct(const cplx<R> *f_dev, cplx<R> *fhat_dev);
@@ -146,7 +179,8 @@ template <typename R>
 void launch_deconv_from_grid(const cplx<R> *g, cplx<R> *fhat, const R *phi_inv, int d,
                              const long *N, const long *n, hipStream_t s);
 template <typename R>
-void launch_fft_axis(cplx<R> *data, const FftAxis<R> &ax, int sign, hipStream_t s);
+void launch_fft_pass(cplx<R> *g, const cplx<R> *fhat_in, cplx<R> *fhat_out, const FftPass<R> &p,
+                     int sign, hipStream_t s);
 template <typename R>
 void launch_interp(const cplx<R> *g, cplx<R> *f, const Engine<R> &e, hipStream_t s);
 template <typename R>

@@ -1,18 +1,34 @@
-// F: the d-dimensional unnormalised DFT of the oversampled grid, done as one
-// in-place pass per axis (the role FFTW plays for the reference:
-// kernel/nfft/nfft.c:6030-6031 plans, FFTW_FORWARD = sign -1 for trafo,
-// FFTW_BACKWARD = sign +1 for adjoint).
+// F: the d-dimensional unnormalised DFT of the oversampled grid (the role
+// FFTW plays for the reference: kernel/nfft/nfft.c:6030-6031 plans,
+// FFTW_FORWARD = sign -1 for trafo, FFTW_BACKWARD = sign +1 for adjoint),
+// done as one pass per dimension with D and D^T fused into the passes at the
+// ends and the zero band of the padded grid pruned:
 //
-// Each workgroup stages `lines_per_block` whole lines of one axis in LDS and
-// runs a self-sorting Stockham schedule over them (mixed radix 2/3/4/5/7/8 and
-// small primes), ping-ponging between two LDS buffers, then writes the lines
-// back.  Stage with radix R after sub-transforms of size Ns:
+//   trafo   passes t = d-1 .. 0.  The first reads f_hat, applies the
+//           deconvolution factors and places the values in the band
+//           (k - N_t/2) mod n_t of each line (D, reference
+//           kernel/nfft/nfft.c:2786-2831); every pass handles only lines whose
+//           not-yet-transformed coordinates lie in their band, and reads only
+//           the band along its own axis (the rest is known to be zero).
+//   adjoint passes t = 0 .. d-1.  Each keeps only the band of its axis (all
+//           D^T reads, nfft.c:2874-2903) and handles only lines whose already
+//           transformed coordinates lie in the band; the last writes f_hat
+//           with the deconvolution factors applied (D^T).
+//
+// Each workgroup stages `lines_per_block` whole lines in LDS and runs a
+// self-sorting Stockham schedule over them in place (mixed radix 2/3/4/5/7/8
+// and small primes; every stage reads its butterflies into registers,
+// synchronises, then writes).  Stage with radix R after sub-transforms of
+// size Ns:
 //   v_r = X[j + r n/R] * w^(r (j mod Ns))  with w = exp(sign 2 pi i / (Ns R))
 //   v   = DFT_R(v)
 //   Y[(j div Ns) Ns R + (j mod Ns) + r Ns] = v_r
 // Twiddles come from a table exp(-2 pi i k / n) built in FP64 on the host.
 //
-// Roofline: HBM-bound; algorithmic bytes per axis pass 2 n_total c.
+// Roofline: HBM-bound.  Algorithmic bytes of the whole F with D fused, for
+// d = 3 and c bytes per complex: N_total c + the band-pruned intermediate
+// reads and writes (N0 N1 n2 + N0 n1 n2 + n_total) c written, the same minus
+// the final n_total read back by the next pass.
 #include "engine.hpp"
 
 namespace nfft_hip {
@@ -121,113 +137,205 @@ __device__ __forceinline__ void dft(cplx<R> *v) {
   else dft_generic<R, RAD, SIGN>(v);
 }
 
-template <typename R, int RAD, int SIGN>
-__device__ void stockham_stage(const cplx<R> *__restrict__ src, cplx<R> *__restrict__ dst, int n,
-                               int nlines, int Ns, const cplx<R> *__restrict__ tw) {
+// Butterflies per thread of one stage: lines * n / RAD over the workgroup.
+template <int CAP, int RAD> constexpr int stage_slots() { return (CAP / RAD + 255) / 256; }
+
+// One in-place Stockham stage over `nlines` lines of length n in LDS.
+template <typename R, int RAD, int SIGN, int CAP>
+__device__ __forceinline__ void stage_inplace(cplx<R> *__restrict__ buf, int n, int nlines, int Ns,
+                                              const cplx<R> *__restrict__ tw) {
+  constexpr int SL = stage_slots<CAP, RAD>();
   const int q = n / RAD;
   const int nbf = nlines * q;
   const int span = Ns * RAD;
   const int twstep = n / span;
-  for (int b = threadIdx.x; b < nbf; b += blockDim.x) {
-    int ll = b / q;
-    int j = b - ll * q;
-    int jm = j % Ns;
-    const cplx<R> *s = src + ll * n + j;
-    cplx<R> v[RAD];
+  cplx<R> v[SL][RAD];
+  int dsto[SL];
 #pragma unroll
-    for (int r = 0; r < RAD; r++) v[r] = s[r * q];
-    if (Ns > 1) {
+  for (int i = 0; i < SL; i++) {
+    const int b = threadIdx.x + i * 256;
+    dsto[i] = -1;
+    if (b < nbf) {
+      const int ll = b / q;
+      const int j = b - ll * q;
+      const int jm = j % Ns;
+      const cplx<R> *src = buf + ll * n + j;
 #pragma unroll
-      for (int r = 1; r < RAD; r++) {
-        cplx<R> w = tw[jm * r * twstep];
-        if (SIGN > 0) w.y = -w.y;
-        v[r] = cmul<R>(v[r], w);
+      for (int r = 0; r < RAD; r++) v[i][r] = src[r * q];
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < RAD; r++) {
+          cplx<R> w = tw[jm * r * twstep];
+          if (SIGN > 0) w.y = -w.y;
+          v[i][r] = cmul<R>(v[i][r], w);
+        }
       }
+      dft<R, RAD, SIGN>(v[i]);
+      dsto[i] = ll * n + (j / Ns) * span + jm;
     }
-    dft<R, RAD, SIGN>(v);
-    cplx<R> *o = dst + ll * n + (j / Ns) * span + jm;
-#pragma unroll
-    for (int r = 0; r < RAD; r++) o[r * Ns] = v[r];
   }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < SL; i++) {
+    if (dsto[i] >= 0) {
+      cplx<R> *o = buf + dsto[i];
+#pragma unroll
+      for (int r = 0; r < RAD; r++) o[r * Ns] = v[i][r];
+    }
+  }
+  __syncthreads();
 }
 
-template <typename R, int SIGN>
-__global__ __launch_bounds__(256) void k_fft_axis(cplx<R> *__restrict__ data, FftAxis<R> ax) {
-  extern __shared__ unsigned char fft_smem[];
-  const int n = ax.len;
-  const int T = ax.lines_per_block;
-  cplx<R> *bufA = reinterpret_cast<cplx<R> *>(fft_smem);
-  cplx<R> *bufB = bufA + (long)T * n;
-  const long L0 = (long)blockIdx.x * T;
-  const int TL = (int)min((long)T, ax.nlines - L0);
-  const int tot = TL * n;
-  const long inner = ax.inner;
+// Band position of grid index k along a dimension (n, N): the f_hat
+// coordinate ks in [0, N) with k = (ks - N/2) mod n, or -1 outside the band.
+__device__ __forceinline__ int band_coord(int k, int n, int N) {
+  const int h = N / 2;
+  if (k < N - h) return k + h;
+  if (k >= n - h) return k - (n - h);
+  return -1;
+}
 
-  // stage TL lines into LDS: line-major for contiguous lines, otherwise
-  // line-minor so that a wave reads TL adjacent lines of one row
+constexpr int kMaxLinesPerBlock = 256;
+
+template <typename R, int SIGN, int CAP>
+__global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
+                                                  const cplx<R> *__restrict__ fin,
+                                                  cplx<R> *__restrict__ fout, FftPass<R> P) {
+  extern __shared__ unsigned char fft_smem[];
+  cplx<R> *buf = reinterpret_cast<cplx<R> *>(fft_smem);
+  __shared__ long s_goff[kMaxLinesPerBlock];
+  __shared__ long s_foff[kMaxLinesPerBlock];
+  __shared__ R s_fac[kMaxLinesPerBlock];
+  const int n = P.len;
+  const long L0 = (long)blockIdx.x * P.lines_per_block;
+  const int TL = (int)min((long)P.lines_per_block, P.nlines - L0);
+  const int tot = TL * n;
+  const bool coeffs = P.in_mode == kFromCoeffs || P.out_mode == kToCoeffs;
+
+  // per line: grid offset, f_hat offset and deconvolution factor of the
+  // other dimensions
+  for (int ll = threadIdx.x; ll < TL; ll += blockDim.x) {
+    long L = L0 + ll;
+    int c[2] = {0, 0};
+    for (int s = P.nod - 1; s >= 0; s--) {
+      c[s] = (int)(L % P.cnt[s]);
+      L /= P.cnt[s];
+    }
+    long go = 0, fo = 0;
+    R fac = R(1);
+    for (int s = 0; s < P.nod; s++) {
+      int k = c[s];
+      if (P.band[s]) {
+        k = c[s] - P.N_od[s] / 2;
+        k += k < 0 ? P.n_od[s] : 0;
+      }
+      go += (long)k * P.gstr[s];
+      if (coeffs) {
+        fo += (long)c[s] * P.fstr[s];
+        fac *= P.phi_od[s][c[s]];
+      }
+    }
+    s_goff[ll] = go;
+    s_foff[ll] = fo;
+    s_fac[ll] = fac;
+  }
+  __syncthreads();
+
+  // stage the lines: a wave reads along the line for the contiguous axis and
+  // across adjacent lines otherwise
+  const bool contiguous = P.stride == 1;
   for (int qd = threadIdx.x; qd < tot; qd += blockDim.x) {
     int ll, k;
-    if (inner == 1) {
+    if (contiguous) {
       ll = qd / n;
       k = qd - ll * n;
     } else {
       k = qd / TL;
       ll = qd - k * TL;
     }
-    long L = L0 + ll;
-    long o = L / inner, i = L - o * inner;
-    bufA[ll * n + k] = data[(o * n + k) * inner + i];
+    cplx<R> v{R(0), R(0)};
+    if (P.in_mode == kFromGrid) {
+      v = g[s_goff[ll] + (long)k * P.stride];
+    } else {
+      const int ks = band_coord(k, n, P.Nband);
+      if (ks >= 0) {
+        if (P.in_mode == kFromGridBand) {
+          v = g[s_goff[ll] + (long)k * P.stride];
+        } else {
+          const cplx<R> u = fin[s_foff[ll] + (long)ks * P.fstr_ax];
+          const R f = s_fac[ll] * P.phi_ax[ks];
+          v = cplx<R>{u.x * f, u.y * f};
+        }
+      }
+    }
+    buf[ll * n + k] = v;
   }
   __syncthreads();
 
-  cplx<R> *src = bufA, *dst = bufB;
   int Ns = 1;
-  for (int s = 0; s < ax.nstages; s++) {
-    const int rad = ax.radix[s];
+  for (int st = 0; st < P.nstages; st++) {
+    const int rad = P.radix[st];
     switch (rad) {
-      case 2: stockham_stage<R, 2, SIGN>(src, dst, n, TL, Ns, ax.twiddle); break;
-      case 3: stockham_stage<R, 3, SIGN>(src, dst, n, TL, Ns, ax.twiddle); break;
-      case 4: stockham_stage<R, 4, SIGN>(src, dst, n, TL, Ns, ax.twiddle); break;
-      case 5: stockham_stage<R, 5, SIGN>(src, dst, n, TL, Ns, ax.twiddle); break;
-      case 7: stockham_stage<R, 7, SIGN>(src, dst, n, TL, Ns, ax.twiddle); break;
-      case 8: stockham_stage<R, 8, SIGN>(src, dst, n, TL, Ns, ax.twiddle); break;
-      case 11: stockham_stage<R, 11, SIGN>(src, dst, n, TL, Ns, ax.twiddle); break;
-      case 13: stockham_stage<R, 13, SIGN>(src, dst, n, TL, Ns, ax.twiddle); break;
+      case 2: stage_inplace<R, 2, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
+      case 3: stage_inplace<R, 3, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
+      case 4: stage_inplace<R, 4, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
+      case 5: stage_inplace<R, 5, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
+      case 7: stage_inplace<R, 7, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
+      case 8: stage_inplace<R, 8, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
+      case 11: stage_inplace<R, 11, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
+      case 13: stage_inplace<R, 13, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
       default: break;  // excluded at plan time
     }
-    __syncthreads();
-    cplx<R> *t = src;
-    src = dst;
-    dst = t;
     Ns *= rad;
   }
 
   for (int qd = threadIdx.x; qd < tot; qd += blockDim.x) {
     int ll, k;
-    if (inner == 1) {
+    if (contiguous) {
       ll = qd / n;
       k = qd - ll * n;
     } else {
       k = qd / TL;
       ll = qd - k * TL;
     }
-    long L = L0 + ll;
-    long o = L / inner, i = L - o * inner;
-    data[(o * n + k) * inner + i] = src[ll * n + k];
+    const cplx<R> v = buf[ll * n + k];
+    if (P.out_mode == kToGrid) {
+      g[s_goff[ll] + (long)k * P.stride] = v;
+    } else {
+      const int ks = band_coord(k, n, P.Nband);
+      if (ks >= 0) {
+        if (P.out_mode == kToGridBand) {
+          g[s_goff[ll] + (long)k * P.stride] = v;
+        } else {
+          const R f = s_fac[ll] * P.phi_ax[ks];
+          fout[s_foff[ll] + (long)ks * P.fstr_ax] = cplx<R>{v.x * f, v.y * f};
+        }
+      }
+    }
   }
 }
 
+// elements of one workgroup's lines
+template <typename R> constexpr int fft_cap() { return sizeof(R) == 4 ? 4096 : 2048; }
+
 template <typename R>
-void launch_fft_axis(cplx<R> *data, const FftAxis<R> &ax, int sign, hipStream_t s) {
-  long blocks = ceil_div(ax.nlines, ax.lines_per_block);
-  size_t lds = 2 * sizeof(cplx<R>) * (size_t)ax.lines_per_block * (size_t)ax.len;
+void launch_fft_pass(cplx<R> *g, const cplx<R> *fhat_in, cplx<R> *fhat_out, const FftPass<R> &p,
+                     int sign, hipStream_t s) {
+  if (p.nlines <= 0) return;
+  const long blocks = ceil_div(p.nlines, p.lines_per_block);
+  const size_t lds = sizeof(cplx<R>) * (size_t)p.lines_per_block * (size_t)p.len;
+  constexpr int CAP = fft_cap<R>();
   if (sign < 0)
-    hipLaunchKernelGGL((k_fft_axis<R, -1>), dim3((unsigned)blocks), dim3(256), lds, s, data, ax);
+    hipLaunchKernelGGL((k_fft_pass<R, -1, CAP>), dim3((unsigned)blocks), dim3(256), lds, s, g,
+                       fhat_in, fhat_out, p);
   else
-    hipLaunchKernelGGL((k_fft_axis<R, +1>), dim3((unsigned)blocks), dim3(256), lds, s, data, ax);
+    hipLaunchKernelGGL((k_fft_pass<R, +1, CAP>), dim3((unsigned)blocks), dim3(256), lds, s, g,
+                       fhat_in, fhat_out, p);
 }
 
-template void launch_fft_axis<float>(cplx<float> *, const FftAxis<float> &, int, hipStream_t);
-template void launch_fft_axis<double>(cplx<double> *, const FftAxis<double> &, int, hipStream_t);
+template void launch_fft_pass<float>(cplx<float> *, const cplx<float> *, cplx<float> *,
+                                     const FftPass<float> &, int, hipStream_t);
+template void launch_fft_pass<double>(cplx<double> *, const cplx<double> *, cplx<double> *,
+                                      const FftPass<double> &, int, hipStream_t);
 
 }  // namespace nfft_hip

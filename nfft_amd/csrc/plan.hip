@@ -227,15 +227,11 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   // private tiles in 2-D); double: wavefronts sharing a tile with ds_add_f64
   spread_variant = variant("NFFT_HIP_SPREAD", sizeof(R) == 4 ? (d == 3 ? 4 : 2) : 1);
 
-  // FFT axis passes
+  // FFT: twiddles per dimension, then the pruned passes of both directions
   const int cap = sizeof(R) == 4 ? 4096 : 2048;  // elements per workgroup
-  long inner = n_total;
   for (int t = 0; t < d; t++) {
     FftAxis<R> ax{};
     ax.len = (int)n[t];
-    inner /= n[t];
-    ax.inner = inner;
-    ax.nlines = n_total / n[t];
     if (n[t] > cap)
       family<R>::fatal("nfft: FFT length n[%d] = %ld exceeds the supported %d", t, n[t], cap);
     std::vector<int> rad;
@@ -243,7 +239,6 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
       family<R>::fatal("nfft: FFT length n[%d] = %ld has a prime factor above 13", t, n[t]);
     ax.nstages = (int)rad.size();
     for (size_t i = 0; i < rad.size(); i++) ax.radix[i] = rad[i];
-    ax.lines_per_block = std::max(1, cap / (int)n[t]);
     std::vector<cplx<R>> tw(n[t]);
     for (long k = 0; k < n[t]; k++) {
       long double a = -2.0L * kPiL * (long double)k / (long double)n[t];
@@ -253,6 +248,58 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     NFFT_HIP_CHECK(R, hipMemcpy(ax.twiddle, tw.data(), sizeof(cplx<R>) * tw.size(),
                                 hipMemcpyHostToDevice));
     axes.push_back(ax);
+  }
+  {
+    long gstride[kMaxDim], fstride[kMaxDim];
+    int phioff[kMaxDim];
+    long gs = 1, fs = 1;
+    for (int t = d - 1; t >= 0; t--) {
+      gstride[t] = gs;
+      fstride[t] = fs;
+      gs *= n[t];
+      fs *= N[t];
+    }
+    int off = 0;
+    for (int t = 0; t < d; t++) {
+      phioff[t] = off;
+      off += (int)N[t];
+    }
+    // band of dimension s on the lines of the pass over t: trafo, s < t (not
+    // yet transformed, zero outside the band); adjoint, s < t (already
+    // cropped to the band)
+    auto make_pass = [&](int t, int in_mode, int out_mode) {
+      FftPass<R> p{};
+      p.len = (int)n[t];
+      p.stride = gstride[t];
+      p.Nband = (int)N[t];
+      p.in_mode = in_mode;
+      p.out_mode = out_mode;
+      p.nod = 0;
+      p.nlines = 1;
+      for (int s = 0; s < d; s++) {
+        if (s == t) continue;
+        const int j = p.nod++;
+        p.band[j] = s < t ? 1 : 0;
+        p.cnt[j] = (int)(p.band[j] ? N[s] : n[s]);
+        p.N_od[j] = (int)N[s];
+        p.n_od[j] = (int)n[s];
+        p.gstr[j] = gstride[s];
+        p.fstr[j] = fstride[s];
+        p.phi_od[j] = phi_inv + phioff[s];
+        p.nlines *= p.cnt[j];
+      }
+      p.fstr_ax = fstride[t];
+      p.phi_ax = phi_inv + phioff[t];
+      p.lines_per_block = (int)std::max<long>(1, std::min<long>(256, cap / n[t]));
+      p.nstages = axes[t].nstages;
+      for (int i = 0; i < p.nstages; i++) p.radix[i] = axes[t].radix[i];
+      p.twiddle = axes[t].twiddle;
+      return p;
+    };
+    for (int t = d - 1; t >= 0; t--)
+      fwd_passes.push_back(make_pass(t, t == d - 1 ? kFromCoeffs : kFromGridBand, kToGrid));
+    for (int t = 0; t < d; t++)
+      bwd_passes.push_back(make_pass(t, kFromGrid, t == d - 1 ? kToCoeffs : kToGridBand));
   }
 }
 
@@ -326,14 +373,14 @@ hipEvent_t *Engine<R>::call_events(int kind) {
 
 // Event layout of one call.  trafo: E0 | D | E1 | FFT | E2 | B | E3 (E4 = E3).
 // adjoint: E0 | B^T spreading | E1 | tile reduction | E2 | FFT | E3 | D^T | E4.
+// D and D^T are fused into the FFT passes, so their intervals are empty.
 template <typename R>
 void Engine<R>::trafo(const cplx<R> *fhat_dev, cplx<R> *f_dev) {
   hipEvent_t *E = call_events(0);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[0], stream));
-  launch_fill_zero<R>(g, n_total, stream);
-  launch_deconv_to_grid<R>(fhat_dev, g, phi_inv, d, N, n, stream);
+  // D runs inside the first FFT pass: the D interval stays empty
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[1], stream));
-  for (auto &ax : axes) launch_fft_axis<R>(g, ax, -1, stream);
+  for (auto &ps : fwd_passes) launch_fft_pass<R>(g, fhat_dev, nullptr, ps, -1, stream);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[2], stream));
   launch_interp<R>(g, f_dev, *this, stream);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[3], stream));
@@ -353,10 +400,25 @@ void Engine<R>::adjoint(const cplx<R> *f_dev, cplx<R> *fhat_dev) {
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[1], stream));
   if (scratch) launch_gather_tiles<R>(*this, g, stream);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[2], stream));
-  for (auto &ax : axes) launch_fft_axis<R>(g, ax, +1, stream);
+  // D^T runs inside the last FFT pass: the D^T interval stays empty
+  for (auto &ps : bwd_passes) launch_fft_pass<R>(g, nullptr, fhat_dev, ps, +1, stream);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[3], stream));
-  launch_deconv_from_grid<R>(g, fhat_dev, phi_inv, d, N, n, stream);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[4], stream));
+  NFFT_HIP_CHECK(R, hipGetLastError());
+}
+
+template <typename R>
+void Engine<R>::spread_to_grid(const cplx<R> *f_dev) {
+  const bool scratch = tile_scratch && sp_nsub > 0;
+  if (!scratch) launch_fill_zero<R>(g, n_total, stream);
+  launch_spread<R>(f_dev, g, *this, stream);
+  if (scratch) launch_gather_tiles<R>(*this, g, stream);
+  NFFT_HIP_CHECK(R, hipGetLastError());
+}
+
+template <typename R>
+void Engine<R>::interp_from_grid(cplx<R> *f_dev) {
+  launch_interp<R>(g, f_dev, *this, stream);
   NFFT_HIP_CHECK(R, hipGetLastError());
 }
 
