@@ -20,8 +20,15 @@ Also reported:
   cpu_baseline  the FP64 CPU oracle on a bounded sample of the same workload
                 (full-grid D + FFT, B on 2^18 of the nodes, scaled to M),
                 single-threaded on the host
+  cgnr          the metric's second part: weighted CGNR to normal-equation
+                residual 1e-6 on held-out 2-D trajectories (N = 256^2,
+                sigma 2, m = 4, FP32; a variable-density spiral and a
+                golden-angle radial set; phantom of random ellipses plus 1%
+                complex noise), unweighted vs Pipe-Menon weights vs the
+                learned weights of nfft_amd/data/dcf_model.npy, wall time
+                including the weight computation
 
-python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-cgnr]
 """
 import argparse
 import json
@@ -80,6 +87,61 @@ def cpu_baseline(N, n, m, x, fh, f, sample):
                        "to M=%d; measured %.1f s of CPU work" % (sample, M, t4 - t0))
 
 
+def cgnr_leg(args):
+    """CGNR time to residual 1e-6: unweighted vs Pipe-Menon vs learned W."""
+    import torch
+    import nfft_amd as nf
+    from nfft_amd import dcf_train as T
+    from nfft_amd import density as D
+    from nfft_amd import workloads as W
+
+    N, tol = 256, 1e-6
+    model_path = T.DEFAULT_MODEL
+    if not os.path.exists(model_path):
+        log("cgnr: no trained model at %s, skipped" % model_path)
+        return {"skipped": "no trained model"}
+    model = D.DcfModel.load(model_path)
+    cases = []
+    for seed in (5000, 5001):  # held out: training uses seeds 0..1999
+        x, desc = T.trajectory(seed, N)
+        p = T.make_plan(x, N)
+        f_true = W.phantom(N, 20, seed=seed)
+        p.f_hat[:] = f_true
+        p.trafo()
+        y = p.f.astype(np.complex128).copy()
+        y += 0.01 * np.sqrt(np.mean(np.abs(y) ** 2)) * W.complex_normal(p.M, seed=seed + 1)
+        res = {"trajectory": list(desc), "M": int(p.M)}
+        for name in ("plain", "pipe_menon", "learned"):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            w = None
+            if name == "pipe_menon":
+                w = D.pipe_menon(p, 20)
+            elif name == "learned":
+                w = model.weights(p).cpu().numpy()
+            t_w = time.perf_counter() - t0
+            it, t_s, r, fh = T.cgnr_time_to(p, y, w, tol=tol, max_iter=args.cgnr_max_iter)
+            err = float(np.linalg.norm(fh.real - f_true) / np.linalg.norm(f_true))
+            res[name] = {"iterations": it, "solve_s": t_s, "weights_s": t_w,
+                         "total_s": t_s + t_w, "residual": r, "reached": bool(r < tol),
+                         "image_rel_err": err}
+            log("cgnr %s %s: %d it, %.1f ms (+%.1f ms weights), residual %.1e, image err %.3f"
+                % (desc[0], name, it, t_s * 1e3, t_w * 1e3, r, err))
+        cases.append(res)
+        p.finalize()
+    model.close()
+    tot = {k: sum(c[k]["total_s"] for c in cases) for k in ("plain", "pipe_menon", "learned")}
+    return {
+        "workload": "2-D N=256^2, sigma=2, m=4, FP32; held-out spiral + golden-angle radial; "
+                    "ellipse phantom + 1% noise; CGNR from 0 to sqrt(z^H z / z0^H z0) < 1e-6",
+        "tol": tol,
+        "cases": cases,
+        "total_s": tot,
+        "speedup_learned_vs_plain": tot["plain"] / tot["learned"] if tot["learned"] > 0 else None,
+        "speedup_pipe_menon_vs_plain": tot["plain"] / tot["pipe_menon"] if tot["pipe_menon"] > 0 else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +150,8 @@ def main():
     ap.add_argument("--spokes", type=int, default=16384)
     ap.add_argument("--samples", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cgnr", action="store_true")
+    ap.add_argument("--cgnr-max-iter", type=int, default=3000)
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes "
@@ -173,7 +237,7 @@ def main():
     b_trafo_ms = stages_ms["trafo_B"]
     b_adj_ms = stages_ms["adjoint_B"]
     if b_adj_ms >= b_trafo_ms:
-        dom_name, dom_ms = "B^T spreading (k_bwave_tile, adjoint)", b_adj_ms
+        dom_name, dom_ms = "B^T spreading (k_spread_runs, adjoint)", b_adj_ms
     else:
         dom_name, dom_ms = "B interpolation (k_interp_tile, trafo)", b_trafo_ms
     achieved = ab["B"] / (dom_ms * 1e-3) / 1e9
@@ -224,6 +288,8 @@ def main():
         "cpu_baseline": cpu,
     }
     p.finalize()
+    if not args.no_cgnr:
+        out["cgnr"] = cgnr_leg(args)
     print(json.dumps(out))
 
 

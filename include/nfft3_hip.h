@@ -20,6 +20,9 @@ extern "C"
 {
 #endif
 
+/* An instantiated hipGraph of batched transforms (see P##graph_trafo). */
+typedef struct nfft_hip_graph_s nfft_hip_graph;
+
 #define NFFT_HIP_DECLARE_EXT(P, PLAN)                                        \
   /* f_dev <- A f_hat_dev (same math as P##trafo) */                          \
   void P##trafo_device(PLAN *ths, const void *f_hat_dev, void *f_dev);       \
@@ -48,10 +51,32 @@ extern "C"
   void P##timing_end(PLAN *ths, double *out);                                \
   /* Number of B-stage work items (sub-problems) built by the last           \
    * precompute, for diagnostics. */                                         \
-  long P##num_subproblems(PLAN *ths);
+  long P##num_subproblems(PLAN *ths);                                       \
+  /* nrhs right-hand sides sharing the plan's nodes (e.g. MRI coils):       \
+   * vector r of f_hat_dev starts at r N_total, of f_dev at r M_total. */    \
+  void P##trafo_batch_device(PLAN *ths, const void *f_hat_dev, void *f_dev, \
+                             int nrhs);                                      \
+  void P##adjoint_batch_device(PLAN *ths, const void *f_dev,                \
+                               void *f_hat_dev, int nrhs);                   \
+  /* Capture the batched transforms of count plans (nrhs each, device       \
+   * arrays f_hat_dev[i] / f_dev[i] of plan i) into one hipGraph.  Plans on  \
+   * different streams run concurrently on replay; the graph replays on     \
+   * plans[0]'s stream.  Node data are built before the capture; stage      \
+   * timing must be off.  Replays read the same device arrays. */           \
+  nfft_hip_graph *P##graph_trafo(PLAN **plans, int count,                   \
+                                 const void **f_hat_dev, void **f_dev,       \
+                                 int nrhs);                                  \
+  nfft_hip_graph *P##graph_adjoint(PLAN **plans, int count,                 \
+                                   const void **f_dev, void **f_hat_dev,     \
+                                   int nrhs);
 
 NFFT_HIP_DECLARE_EXT(nfftf_, nfftf_plan)
 NFFT_HIP_DECLARE_EXT(nfft_, nfft_plan)
+
+/* Replay a captured graph (enqueued, no wait), wait for it, release it. */
+void nfft_hip_graph_launch(nfft_hip_graph *graph);
+void nfft_hip_graph_synchronize(nfft_hip_graph *graph);
+void nfft_hip_graph_destroy(nfft_hip_graph *graph);
 
 /* CGNR on one of this library's plans keeps all solver vectors on the GPU.
  * The sync level says what is copied back to the host plan fields after

@@ -287,6 +287,14 @@ class Plan:
                 "adjoint_B", "adjoint_Bred", "adjoint_FFT", "adjoint_D"]
         return dict(zip(keys, list(out)))
 
+    def trafo_batch_device(self, f_hat_ptr, f_ptr, nrhs):
+        self._fn("trafo_batch_device")(self.pp, ctypes.c_void_p(f_hat_ptr), ctypes.c_void_p(f_ptr),
+                                       int(nrhs))
+
+    def adjoint_batch_device(self, f_ptr, f_hat_ptr, nrhs):
+        self._fn("adjoint_batch_device")(self.pp, ctypes.c_void_p(f_ptr), ctypes.c_void_p(f_hat_ptr),
+                                         int(nrhs))
+
     def num_subproblems(self):
         return self._fn("num_subproblems")(self.pp)
 
@@ -364,6 +372,9 @@ class Solver:
         fn.argtypes = [type(self.sp), ctypes.c_int]
         fn(self.sp, int(level))
 
+    def sync(self):
+        getattr(self.L, self.pre + "sync_complex")(self.sp)
+
     def finalize(self):
         if self.alive:
             getattr(self.L, self.pre + "finalize_complex")(self.sp)
@@ -383,3 +394,42 @@ def _next_pow2(x):
     while p < x:
         p <<= 1
     return p
+
+
+class Graph:
+    """hipGraph of the batched transforms of several plans (nfftf_graph_trafo /
+    nfftf_graph_adjoint and nfft_hip_graph_launch / _synchronize / _destroy,
+    include/nfft3_hip.h).  ``inputs`` / ``outputs`` are device pointers, one
+    per plan, each holding ``nrhs`` vectors."""
+
+    def __init__(self, plans, inputs, outputs, nrhs=1, adjoint=False):
+        self.L = lib()
+        prec = plans[0].precision
+        assert all(p.precision == prec for p in plans)
+        pre = "nfft_" if prec == "double" else "nfftf_"
+        n = len(plans)
+        PP = ctypes.c_void_p * n
+        pl = PP(*[ctypes.cast(p.pp, ctypes.c_void_p) for p in plans])
+        ins = PP(*[ctypes.c_void_p(int(a)) for a in inputs])
+        outs = PP(*[ctypes.c_void_p(int(a)) for a in outputs])
+        fn = getattr(self.L, pre + ("graph_adjoint" if adjoint else "graph_trafo"))
+        fn.restype = ctypes.c_void_p
+        self.h = fn(pl, n, ins, outs, int(nrhs))
+        self.plans = plans  # keep the plans alive while the graph exists
+
+    def launch(self):
+        self.L.nfft_hip_graph_launch(ctypes.c_void_p(self.h))
+
+    def synchronize(self):
+        self.L.nfft_hip_graph_synchronize(ctypes.c_void_p(self.h))
+
+    def destroy(self):
+        if self.h:
+            self.L.nfft_hip_graph_destroy(ctypes.c_void_p(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -20,6 +20,12 @@
 #include "../../include/nfft3_hip.h"
 #include "engine.hpp"
 
+// instantiated hipGraph of batched transforms (include/nfft3_hip.h)
+struct nfft_hip_graph_s {
+  hipGraphExec_t exec;
+  hipStream_t stream;
+};
+
 namespace nfft_hip {
 
 // ------------------------------------------------------------------ helpers
@@ -764,6 +770,81 @@ struct PlanApi {
     if (e->timing) store_times(p, e, true);
   }
 
+  // nrhs transforms sharing the plan's nodes: vectors r of f_hat_dev / f_dev
+  // at offsets r N_total / r M_total, enqueued back to back
+  static void trafo_batch_device(PLAN *p, const void *fh, void *f, int nrhs) {
+    for (int r = 0; r < nrhs; r++)
+      trafo_device(p, (const cplx<R> *)fh + (long)r * p->N_total, (cplx<R> *)f + (long)r * p->M_total);
+  }
+  static void adjoint_batch_device(PLAN *p, const void *f, void *fh, int nrhs) {
+    for (int r = 0; r < nrhs; r++)
+      adjoint_device(p, (const cplx<R> *)f + (long)r * p->M_total, (cplx<R> *)fh + (long)r * p->N_total);
+  }
+
+  // One hipGraph for the batched transforms of `count` plans: captured on
+  // plans[0]'s stream, each other plan's stream forks from it and joins it
+  // again, so plans on different streams run concurrently on replay.  Node
+  // data are built before the capture; nothing in the captured work
+  // allocates, copies from the host or clears memory with memset.
+  static nfft_hip_graph *capture(PLAN **plans, int count, const void **in, void **out, int nrhs,
+                                 bool adjoint) {
+    if (count < 1) family<R>::fatal("nfft: a graph needs at least one plan");
+    for (int i = 0; i < count; i++) {
+      Engine<R> *e = eng(plans[i]);
+      if (e->timing || e->deferred)
+        family<R>::fatal("nfft: switch stage timing off before capturing a graph");
+      if (e->direct_only) {
+        if (!e->nodes_ready) precompute(plans[i]);
+      } else {
+        ensure_nodes(plans[i], e);
+      }
+    }
+    hipStream_t s0 = eng(plans[0])->stream;
+    NFFT_HIP_CHECK(R, hipStreamSynchronize(s0));
+    std::vector<hipEvent_t> evs;
+    auto new_event = [&]() {
+      hipEvent_t ev;
+      NFFT_HIP_CHECK(R, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      evs.push_back(ev);
+      return ev;
+    };
+    hipEvent_t fork = new_event();
+    NFFT_HIP_CHECK(R, hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+    NFFT_HIP_CHECK(R, hipEventRecord(fork, s0));
+    for (int i = 0; i < count; i++) {
+      PLAN *p = plans[i];
+      Engine<R> *e = eng(p);
+      hipStream_t si = e->stream;
+      if (si != s0) NFFT_HIP_CHECK(R, hipStreamWaitEvent(si, fork, 0));
+      for (int r = 0; r < nrhs; r++) {
+        if (!adjoint) {
+          const cplx<R> *fh = (const cplx<R> *)in[i] + (long)r * p->N_total;
+          cplx<R> *f = (cplx<R> *)out[i] + (long)r * p->M_total;
+          if (e->direct_only) e->trafo_direct(fh, f);
+          else e->trafo(fh, f);
+        } else {
+          const cplx<R> *f = (const cplx<R> *)in[i] + (long)r * p->M_total;
+          cplx<R> *fh = (cplx<R> *)out[i] + (long)r * p->N_total;
+          if (e->direct_only) e->adjoint_direct(f, fh);
+          else e->adjoint(f, fh);
+        }
+      }
+      if (si != s0) {
+        hipEvent_t join = new_event();
+        NFFT_HIP_CHECK(R, hipEventRecord(join, si));
+        NFFT_HIP_CHECK(R, hipStreamWaitEvent(s0, join, 0));
+      }
+    }
+    hipGraph_t graph;
+    NFFT_HIP_CHECK(R, hipStreamEndCapture(s0, &graph));
+    auto *h = new nfft_hip_graph();
+    h->stream = s0;
+    NFFT_HIP_CHECK(R, hipGraphInstantiate(&h->exec, graph, nullptr, nullptr, 0));
+    NFFT_HIP_CHECK(R, hipGraphDestroy(graph));
+    for (hipEvent_t ev : evs) hipEventDestroy(ev);
+    return h;
+  }
+
   // reference X(check) (kernel/nfft/nfft.c:6169-6207)
   static const char *check(PLAN *p) {
     if (!p->f) return "Member f not initialized.";
@@ -868,10 +949,39 @@ using nfft_hip::PlanApi;
   void P##timing_begin(PLAN *p) { PlanApi<R, PLAN>::eng(p)->timing_begin(); }                 \
   void P##timing_end(PLAN *p, double *out) { PlanApi<R, PLAN>::eng(p)->timing_end(out); }     \
   long P##num_subproblems(PLAN *p) { return PlanApi<R, PLAN>::num_subproblems(p); }           \
+  void P##trafo_batch_device(PLAN *p, const void *fh, void *f, int nrhs) {                    \
+    PlanApi<R, PLAN>::trafo_batch_device(p, fh, f, nrhs);                                     \
+  }                                                                                           \
+  void P##adjoint_batch_device(PLAN *p, const void *f, void *fh, int nrhs) {                  \
+    PlanApi<R, PLAN>::adjoint_batch_device(p, f, fh, nrhs);                                   \
+  }                                                                                           \
+  nfft_hip_graph *P##graph_trafo(PLAN **plans, int count, const void **fh, void **f,          \
+                                 int nrhs) {                                                  \
+    return PlanApi<R, PLAN>::capture(plans, count, fh, f, nrhs, false);                       \
+  }                                                                                           \
+  nfft_hip_graph *P##graph_adjoint(PLAN **plans, int count, const void **f, void **fh,        \
+                                   int nrhs) {                                                \
+    return PlanApi<R, PLAN>::capture(plans, count, f, fh, nrhs, true);                        \
+  }                                                                                           \
   }
 
 NFFT_HIP_EXPORT_PLAN(nfftf_, float, nfftf_plan)
 NFFT_HIP_EXPORT_PLAN(nfft_, double, nfft_plan)
+
+extern "C" {
+void nfft_hip_graph_launch(nfft_hip_graph *g) {
+  NFFT_HIP_CHECK(double, hipGraphLaunch(g->exec, g->stream));
+}
+void nfft_hip_graph_synchronize(nfft_hip_graph *g) {
+  NFFT_HIP_CHECK(double, hipStreamSynchronize(g->stream));
+}
+void nfft_hip_graph_destroy(nfft_hip_graph *g) {
+  if (!g) return;
+  hipStreamSynchronize(g->stream);
+  hipGraphExecDestroy(g->exec);
+  delete g;
+}
+}
 
 namespace nfft_hip {
 

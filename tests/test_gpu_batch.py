@@ -1,0 +1,93 @@
+"""Batched transforms through the C ABI: many right-hand sides on one node
+set (trafo_batch_device / adjoint_batch_device) and several plans in one
+hipGraph (graph_trafo / graph_adjoint).  Both run the same kernels as the
+single calls, so results must be bit-identical to them; graph replays are
+checked one by one, with eager work and synchronisations between replays
+and new input values written before each replay."""
+import numpy as np
+import pytest
+
+import nfft_amd as nf
+from nfft_amd import workloads as W
+
+torch = pytest.importorskip("torch")
+
+FLAGS = nf.PRE_PHI_HUT | nf.PRE_PSI | nf.MALLOC_X | nf.MALLOC_F_HAT | nf.MALLOC_F | nf.FFTW_INIT | nf.FFT_OUT_OF_PLACE
+
+
+def plan(N, x, precision="single", m=4):
+    p = nf.Plan(N, x.shape[0], n=[2 * t for t in N], m=m, flags=FLAGS, precision=precision)
+    p.x[:] = x
+    p.precompute()
+    return p
+
+
+def cdt(precision):
+    return torch.complex64 if precision == "single" else torch.complex128
+
+
+def rand_dev(n, seed, precision):
+    return torch.from_numpy(W.complex_normal(n, seed=seed)).to(cdt(precision)).cuda()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_batch_matches_single_calls(precision):
+    N = [32, 32]
+    x = W.golden_radial(50, 64)
+    p = plan(N, x, precision)
+    try:
+        nrhs, NN, M = 5, 32 * 32, p.M
+        fh = rand_dev(nrhs * NN, 1, precision)
+        f = torch.empty(nrhs * M, dtype=cdt(precision), device="cuda")
+        p.trafo_batch_device(fh.data_ptr(), f.data_ptr(), nrhs)
+        fin = rand_dev(nrhs * M, 2, precision)
+        fho = torch.empty(nrhs * NN, dtype=cdt(precision), device="cuda")
+        p.adjoint_batch_device(fin.data_ptr(), fho.data_ptr(), nrhs)
+        p.synchronize()
+        for r in range(nrhs):
+            one = torch.empty(M, dtype=cdt(precision), device="cuda")
+            p.trafo_device(fh[r * NN:].data_ptr(), one.data_ptr())
+            oneh = torch.empty(NN, dtype=cdt(precision), device="cuda")
+            p.adjoint_device(fin[r * M:].data_ptr(), oneh.data_ptr())
+            p.synchronize()
+            assert torch.equal(one, f[r * M:(r + 1) * M])
+            assert torch.equal(oneh, fho[r * NN:(r + 1) * NN])
+    finally:
+        p.finalize()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_graph_of_many_plans_replays(adjoint):
+    sizes = [(16, 16), (32, 32), (24, 40)]
+    Ms = [700, 5000, 2100]
+    plans = [plan(list(N), W.uniform(M, 2, seed=10 + i)) for i, (N, M) in enumerate(zip(sizes, Ms))]
+    try:
+        nrhs = 3
+        NNs = [int(np.prod(N)) for N in sizes]
+        nin = [(Ms[i] if adjoint else NNs[i]) * nrhs for i in range(3)]
+        nout = [(NNs[i] if adjoint else Ms[i]) * nrhs for i in range(3)]
+        ins = [rand_dev(nin[i], 20 + i, "single") for i in range(3)]
+        outs = [torch.zeros(nout[i], dtype=torch.complex64, device="cuda") for i in range(3)]
+        g = nf.Graph(plans, [t.data_ptr() for t in ins], [t.data_ptr() for t in outs], nrhs, adjoint)
+        try:
+            for rep in range(4):
+                for i in range(3):  # new input values before every replay
+                    ins[i].copy_(rand_dev(nin[i], 100 * rep + i, "single"))
+                torch.cuda.synchronize()
+                g.launch()
+                g.synchronize()
+                for i, p in enumerate(plans):
+                    ref = torch.empty_like(outs[i])
+                    if adjoint:
+                        p.adjoint_batch_device(ins[i].data_ptr(), ref.data_ptr(), nrhs)
+                    else:
+                        p.trafo_batch_device(ins[i].data_ptr(), ref.data_ptr(), nrhs)
+                    p.synchronize()
+                    assert torch.equal(ref, outs[i]), (rep, i)
+        finally:
+            g.destroy()
+    finally:
+        for p in plans:
+            p.finalize()
