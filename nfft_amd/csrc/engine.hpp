@@ -30,6 +30,9 @@ struct BinGeom {
   int nb[kMaxDim];  // bins per dimension
   int E[kMaxDim];   // LDS tile edge: bs + 2m + 1
   long M;
+  // spreading tiles go to scratch in zone order (see spread_impl.hpp):
+  // needs bs_t >= m + 1 and bs_t dividing n_t
+  int zoned;
 };
 
 // Kaiser-Bessel window of one dimension, scaled by 1/phi(0) so that the B

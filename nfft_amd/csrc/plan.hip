@@ -216,6 +216,15 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
       fits = fits && geom.E[t] <= n[t];
     }
     tile_scratch = fits && tb <= 64 * 1024;
+    bool zoned = tile_scratch;
+    long core = 1;
+    for (int t = 0; t < d; t++) {
+      zoned = zoned && geom.bs[t] >= m + 1 && n[t] % geom.bs[t] == 0;
+      core *= geom.bs[t];
+    }
+    zoned = zoned && core <= 4096;  // the zone reduction's cells per workgroup
+    if (const char *z = std::getenv("NFFT_HIP_ZONES")) zoned = zoned && std::strcmp(z, "0") != 0;
+    geom.zoned = zoned ? 1 : 0;
   }
   auto variant = [](const char *name, int dflt) {
     const char *v = std::getenv(name);

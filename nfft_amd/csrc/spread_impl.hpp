@@ -68,6 +68,67 @@ __device__ __forceinline__ long region_to_grid(int q, const int (&lo)[D], const 
   return gi;
 }
 
+// Spreading tiles in scratch.  Linear order is the LDS order.  Zone order
+// (BinGeom::zoned) splits every dimension of the tile into the part that
+// belongs to the left neighbour bin's core (m cells), the own core (bs) and
+// the right neighbour's core (m + 1), and stores the 3^d zones one after the
+// other (row-major over zone indices, each zone row-major inside), so the
+// tile reduction reads each contribution as one contiguous block.
+template <int D>
+__device__ __forceinline__ void zone_dims(const BinGeom &gg, int t, int z, int &start, int &size) {
+  start = z == 0 ? 0 : (z == 1 ? gg.m : gg.m + gg.bs[t]);
+  size = z == 0 ? gg.m : (z == 1 ? gg.bs[t] : gg.m + 1);
+}
+
+// Writes the block's LDS tile (values from get(q), q the LDS index) to
+// scratch, or adds it into the grid with atomics when there is no scratch.
+template <typename R, int D, typename Get>
+__device__ __forceinline__ void flush_tile(Get get, cplx<R> *__restrict__ tiles,
+                                           cplx<R> *__restrict__ g, const int (&lo)[D],
+                                           const BinGeom &gg, int tot, int tid, int nthr) {
+  if (tiles && gg.zoned) {
+    cplx<R> *out = tiles + (long)blockIdx.x * tot;
+    int zcount = 1;
+#pragma unroll
+    for (int t = 0; t < D; t++) zcount *= 3;
+    long o = 0;
+    for (int zz = 0; zz < zcount; zz++) {
+      int st[3] = {0, 0, 0}, sz[3] = {1, 1, 1};
+      int r = zz;
+#pragma unroll
+      for (int t = D - 1; t >= 0; t--) {
+        zone_dims<D>(gg, t, r % 3, st[t], sz[t]);
+        r /= 3;
+      }
+      int cnt = 1;
+#pragma unroll
+      for (int t = 0; t < D; t++) cnt *= sz[t];
+      for (int q = tid; q < cnt; q += nthr) {
+        int rem = q, src = 0, stride = 1;
+#pragma unroll
+        for (int t = D - 1; t >= 0; t--) {
+          const int i = rem % sz[t];
+          rem /= sz[t];
+          src += (st[t] + i) * stride;
+          stride *= gg.E[t];
+        }
+        out[o + q] = get(src);
+      }
+      o += cnt;
+    }
+    return;
+  }
+  for (int q = tid; q < tot; q += nthr) {
+    const cplx<R> v = get(q);
+    if (tiles) tiles[(long)blockIdx.x * tot + q] = v;
+    else if (v.x != R(0) || v.y != R(0)) {
+      const long gi = region_to_grid<D>(q, lo, gg);
+      unsafeAtomicAdd(&g[gi].x, v.x);
+      unsafeAtomicAdd(&g[gi].y, v.y);
+    }
+  }
+}
+
 template <typename R>
 __device__ __forceinline__ void global_add(cplx<R> *a, R vr, R vi) {
   unsafeAtomicAdd(&a->x, vr);
@@ -333,11 +394,8 @@ __global__ __launch_bounds__(256) void k_spread_tile(const cplx<R> *__restrict__
     }
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < tot; q += blockDim.x) {
-    const cplx<R> v = acc.load(tile + q);
-    if (tiles) tiles[(long)blockIdx.x * tot + q] = v;
-    else if (v.x != R(0) || v.y != R(0)) global_add<R>(g + region_to_grid<D>(q, lo, gg), v.x, v.y);
-  }
+  flush_tile<R, D>([&](int q) { return acc.load(tile + q); }, tiles, g, lo, gg, tot, threadIdx.x,
+                   blockDim.x);
 }
 
 // ============================================================ wave per node
@@ -488,11 +546,8 @@ __global__ __launch_bounds__(256) void k_bwave_tile(const cplx<R> *__restrict__ 
   }
   if constexpr (SPREAD) {
     __syncthreads();
-    for (int q = threadIdx.x; q < tot; q += blockDim.x) {
-      const cplx<R> v = acc.load(atile + q);
-      if (tiles) tiles[(long)blockIdx.x * tot + q] = v;
-      else if (v.x != R(0) || v.y != R(0)) global_add<R>(dst + region_to_grid<D>(q, lo, gg), v.x, v.y);
-    }
+    flush_tile<R, D>([&](int q) { return acc.load(atile + q); }, tiles, dst, lo, gg, tot,
+                     threadIdx.x, blockDim.x);
   }
 }
 
@@ -617,11 +672,7 @@ __global__ __launch_bounds__(64) void k_spread_wave_priv(const cplx<R> *__restri
     cf = nf;
   }
   __syncthreads();
-  for (int q = lane; q < tot; q += 64) {
-    const cplx<R> v = tile[q];
-    if (tiles) tiles[(long)blockIdx.x * tot + q] = v;
-    else if (v.x != R(0) || v.y != R(0)) global_add<R>(g + region_to_grid<D>(q, lo, gg), v.x, v.y);
-  }
+  flush_tile<R, D>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, lane, 64);
 }
 
 // ================================================== row-partitioned B^T, d = 3
@@ -757,11 +808,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_rows(const cplx<R> *__restri
     cf = nf;
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < tot; q += blockDim.x) {
-    const cplx<R> v = tile[q];
-    if (tiles) tiles[(long)blockIdx.x * tot + q] = v;
-    else if (v.x != R(0) || v.y != R(0)) global_add<R>(g + region_to_grid<3>(q, lo, gg), v.x, v.y);
-  }
+  flush_tile<R, 3>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, threadIdx.x, blockDim.x);
 }
 
 }  // namespace nfft_hip
@@ -935,11 +982,8 @@ __global__ __launch_bounds__(256) void k_spread_rt(const cplx<R> *__restrict__ f
   }
   if constexpr (TILE) {
     __syncthreads();
-    for (int q = threadIdx.x; q < tot; q += blockDim.x) {
-      const cplx<R> v = acc.load(tile + q);
-      if (tiles) tiles[(long)blockIdx.x * tot + q] = v;
-      else if (v.x != R(0) || v.y != R(0)) global_add<R>(g + region_to_grid<D>(q, lo, gg), v.x, v.y);
-    }
+    flush_tile<R, D>([&](int q) { return acc.load(tile + q); }, tiles, g, lo, gg, tot,
+                     threadIdx.x, blockDim.x);
   }
 }
 
@@ -1039,10 +1083,140 @@ __global__ __launch_bounds__(256) void k_gather_tiles(const cplx<R> *__restrict_
   }
 }
 
+// Tile reduction for zone-ordered tiles: bin b's core receives, per
+// dimension, the right zone of its left neighbour (core cells [0, m+1)),
+// its own core zone, and the left zone of its right neighbour (core cells
+// [bs-m, bs)); every work item of each of the 3^d neighbour bins contributes
+// one contiguous zone.  Threads own fixed core cells and sum in registers;
+// each grid cell is written once.
+constexpr int kZoneCellsPerThread = 16;
+template <typename R, int D>
+__global__ __launch_bounds__(256) void k_gather_zones(const cplx<R> *__restrict__ tiles,
+                                                      const int *__restrict__ sp_first,
+                                                      const int *__restrict__ sp_count,
+                                                      cplx<R> *__restrict__ g, BinGeom gg) {
+  constexpr int NZ = D == 1 ? 3 : (D == 2 ? 9 : 27);
+  __shared__ long s_base[NZ];   // tile offset of the zone in item 0 of the neighbour
+  __shared__ int s_first[NZ], s_cnt[NZ];
+  __shared__ int s_off[NZ][3], s_size[NZ][3];
+  int bc[3] = {0, 0, 0};
+  {
+    int r = blockIdx.x;
+#pragma unroll
+    for (int t = D - 1; t >= 0; t--) {
+      bc[t] = r % gg.nb[t];
+      r /= gg.nb[t];
+    }
+  }
+  long tot = 1;
+#pragma unroll
+  for (int t = 0; t < D; t++) tot *= gg.E[t];
+  if (threadIdx.x < NZ) {
+    // neighbour offset delta_t = digit - 1; it contributes zone 2 - digit
+    int dig[3] = {1, 1, 1};
+    int r = threadIdx.x;
+#pragma unroll
+    for (int t = D - 1; t >= 0; t--) {
+      dig[t] = r % 3;
+      r /= 3;
+    }
+    int nbl = 0, zidx = 0;
+    int zsz[3] = {1, 1, 1};
+#pragma unroll
+    for (int t = 0; t < D; t++) {
+      const int nb = (bc[t] + dig[t] - 1 + gg.nb[t]) % gg.nb[t];
+      nbl = nbl * gg.nb[t] + nb;
+      const int z = 2 - dig[t];
+      int st;
+      zone_dims<D>(gg, t, z, st, zsz[t]);
+      zidx = zidx * 3 + z;
+      s_size[threadIdx.x][t] = zsz[t];
+      s_off[threadIdx.x][t] = dig[t] == 0 ? 0 : (dig[t] == 1 ? 0 : gg.bs[t] - gg.m);
+    }
+    // offset of zone zidx: sum of the sizes of the zones before it
+    long base = 0;
+    for (int q = 0; q < zidx; q++) {
+      int rq = q, c = 1;
+      for (int t = D - 1; t >= 0; t--) {
+        int st, sz;
+        zone_dims<D>(gg, t, rq % 3, st, sz);
+        rq /= 3;
+        c *= sz;
+      }
+      base += c;
+    }
+    s_base[threadIdx.x] = base;
+    s_first[threadIdx.x] = sp_first[nbl];
+    s_cnt[threadIdx.x] = sp_count[nbl];
+  }
+  __syncthreads();
+  int core = 1;
+#pragma unroll
+  for (int t = 0; t < D; t++) core *= gg.bs[t];
+  int kc[kZoneCellsPerThread][3];
+  R ar[kZoneCellsPerThread], ai[kZoneCellsPerThread];
+#pragma unroll
+  for (int i = 0; i < kZoneCellsPerThread; i++) {
+    const int q = threadIdx.x + i * blockDim.x;
+    int rem = q < core ? q : 0;
+    kc[i][0] = kc[i][1] = kc[i][2] = 0;
+#pragma unroll
+    for (int t = D - 1; t >= 0; t--) {
+      kc[i][t] = rem % gg.bs[t];
+      rem /= gg.bs[t];
+    }
+    ar[i] = ai[i] = R(0);
+  }
+  for (int z = 0; z < NZ; z++) {
+    const int cnt = s_cnt[z];
+    if (cnt == 0) continue;
+    const cplx<R> *zb = tiles + (long)s_first[z] * tot + s_base[z];
+#pragma unroll
+    for (int i = 0; i < kZoneCellsPerThread; i++) {
+      const int q = threadIdx.x + i * blockDim.x;
+      if (q >= core) break;
+      bool in = true;
+      int idx = 0;
+#pragma unroll
+      for (int t = 0; t < D; t++) {
+        const int u = kc[i][t] - s_off[z][t];
+        in = in && u >= 0 && u < s_size[z][t];
+        idx = idx * s_size[z][t] + u;
+      }
+      if (!in) continue;
+      for (int c = 0; c < cnt; c++) {
+        const cplx<R> v = zb[(long)c * tot + idx];
+        ar[i] += v.x;
+        ai[i] += v.y;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kZoneCellsPerThread; i++) {
+    const int q = threadIdx.x + i * blockDim.x;
+    if (q >= core) break;
+    long gi = 0;
+#pragma unroll
+    for (int t = 0; t < D; t++) gi = gi * gg.n[t] + bc[t] * gg.bs[t] + kc[i][t];
+    g[gi] = cplx<R>{ar[i], ai[i]};
+  }
+}
+
 template <typename R>
 void launch_gather_tiles(const Engine<R> &e, cplx<R> *g, hipStream_t s) {
   long nbins = 1;
   for (int t = 0; t < e.geom.d; t++) nbins *= e.geom.nb[t];
+  if (e.geom.zoned) {
+    long core = 1;
+    for (int t = 0; t < e.geom.d; t++) core *= e.geom.bs[t];
+    if (core <= 256L * kZoneCellsPerThread) {
+      auto fz = e.geom.d == 1 ? k_gather_zones<R, 1> : e.geom.d == 2 ? k_gather_zones<R, 2>
+                                                                     : k_gather_zones<R, 3>;
+      hipLaunchKernelGGL(fz, dim3((unsigned)nbins), dim3(256), 0, s, e.tiles, e.sp_first,
+                         e.sp_count, g, e.geom);
+      return;
+    }
+  }
   auto fn = e.geom.d == 1 ? k_gather_tiles<R, 1> : e.geom.d == 2 ? k_gather_tiles<R, 2>
                                                                  : k_gather_tiles<R, 3>;
   hipLaunchKernelGGL(fn, dim3((unsigned)nbins), dim3(256), 0, s, e.tiles, e.sp_first, e.sp_count,

@@ -169,14 +169,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_rows2(const cplx<R> *__restr
     cf = nf;
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < tot; q += blockDim.x) {
-    const cplx<R> v = tile[q];
-    if (tiles) tiles[(long)blockIdx.x * tot + q] = v;
-    else if (v.x != R(0) || v.y != R(0)) {
-      unsafeAtomicAdd(&g[region_to_grid<3>(q, lo, gg)].x, v.x);
-      unsafeAtomicAdd(&g[region_to_grid<3>(q, lo, gg)].y, v.y);
-    }
-  }
+  flush_tile<R, 3>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, threadIdx.x, blockDim.x);
 }
 
 }  // namespace nfft_hip

@@ -234,14 +234,8 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
   }
   if (cur >= 0) flush();
   __syncthreads();
-  for (int q = threadIdx.x; q < tot; q += blockDim.x) {
-    const float2 v = tile[q];
-    if (tiles) tiles[(long)blockIdx.x * tot + q] = v;
-    else if (v.x != 0.f || v.y != 0.f) {
-      unsafeAtomicAdd(&g[region_to_grid<3>(q, lo, gg)].x, v.x);
-      unsafeAtomicAdd(&g[region_to_grid<3>(q, lo, gg)].y, v.y);
-    }
-  }
+  flush_tile<float, 3>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, threadIdx.x,
+                       blockDim.x);
 }
 
 }  // namespace nfft_hip
