@@ -149,6 +149,11 @@ template <typename R> struct Engine {
   void destroy();
   void upload_nodes(const R *x_host);
   void build_nodes();  // from xdev
+  // FFTW_MEASURE plans: time the available B and B^T kernel shapes on the
+  // plan's own nodes once and keep the fastest (kernel variant overrides in
+  // the environment win)
+  bool tuned = false;
+  void autotune();
   void trafo(const cplx<R> *fhat_dev, cplx<R> *f_dev);
   // B^T alone: g <- B^T f (spreading, then the tile reduction or a zeroed
   // grid with atomics); B alone: f <- B g.  Node data must be built.

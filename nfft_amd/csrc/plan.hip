@@ -423,6 +423,59 @@ void Engine<R>::adjoint(const cplx<R> *f_dev, cplx<R> *fhat_dev) {
 }
 
 template <typename R>
+void Engine<R>::autotune() {
+  tuned = true;
+  if (direct_only || M == 0 || timing || deferred) return;
+  hipEvent_t a, b;
+  NFFT_HIP_CHECK(R, hipEventCreate(&a));
+  NFFT_HIP_CHECK(R, hipEventCreate(&b));
+  auto time_of = [&](auto &&run) {
+    run();  // warm-up
+    NFFT_HIP_CHECK(R, hipEventRecord(a, stream));
+    for (int rep = 0; rep < 2; rep++) run();
+    NFFT_HIP_CHECK(R, hipEventRecord(b, stream));
+    NFFT_HIP_CHECK(R, hipEventSynchronize(b));
+    float ms = 0.f;
+    NFFT_HIP_CHECK(R, hipEventElapsedTime(&ms, a, b));
+    return ms;
+  };
+  // inputs: the device image of f (its values do not change the timing)
+  if (!std::getenv("NFFT_HIP_SPREAD")) {
+    // float candidates accumulate in plain FP32 (the fixed-point LDS
+    // variants are left out); double ones use ds_add_f64
+    std::vector<int> cand;
+    if (sizeof(R) == 4 && d == 3) cand = {4, 3};
+    else if (sizeof(R) == 8 && d >= 2) cand = {1, 0};
+    float best = 1e30f;
+    int pick = spread_variant;
+    for (int v : cand) {
+      spread_variant = v;
+      const float ms = time_of([&]() { spread_to_grid(f); });
+      if (ms < best) {
+        best = ms;
+        pick = v;
+      }
+    }
+    spread_variant = pick;
+  }
+  if (!std::getenv("NFFT_HIP_INTERP") && d >= 2) {
+    float best = 1e30f;
+    int pick = interp_variant;
+    for (int v : {0, 1}) {
+      interp_variant = v;
+      const float ms = time_of([&]() { interp_from_grid(f); });
+      if (ms < best) {
+        best = ms;
+        pick = v;
+      }
+    }
+    interp_variant = pick;
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+}
+
+template <typename R>
 void Engine<R>::spread_to_grid(const cplx<R> *f_dev) {
   const bool scratch = tile_scratch && sp_nsub > 0;
   if (!scratch) launch_fill_zero<R>(g, n_total, stream);
@@ -674,6 +727,9 @@ struct PlanApi {
     Engine<R> *e = eng(p);
     e->upload_nodes(p->x);
     e->build_nodes();
+    // FFTW_MEASURE (no FFTW_ESTIMATE bit) asks for measured plans: here the
+    // kernel shapes of B and B^T are timed once on the plan's nodes
+    if (!(p->fftw_flags & FFTW_ESTIMATE) && !e->tuned) e->autotune();
   }
 
   static void ensure_nodes(PLAN *p, Engine<R> *e) {

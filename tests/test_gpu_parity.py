@@ -229,3 +229,33 @@ def test_c_caller_runs(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0 and "OK" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("case", [c for c in CASES if len(c[1]) >= 2], ids=[c[0] for c in CASES if len(c[1]) >= 2])
+def test_measured_plan_vs_oracle(case, precision):
+    """FFTW_MEASURE plans time the B / B^T kernel shapes at precompute and keep
+    the fastest; results stay within the same tolerances."""
+    name, N, gen, m = case
+    x = gen()
+    p = nf.Plan(N, x.shape[0], n=[2 * O.next_power_of_2(t) for t in N], m=m, flags=FLAGS,
+                precision=precision, fftw_flags=nf.FFTW_MEASURE)
+    try:
+        p.x[:] = x
+        p.precompute()
+        xr = p.x.astype(np.float64).copy()
+        p.f_hat[:] = W.complex_normal(int(np.prod(N)), seed=71)
+        fhr = p.f_hat.astype(np.complex128).copy()
+        p.trafo()
+        fg = p.f.astype(np.complex128).copy()
+        p.f[:] = W.complex_normal(x.shape[0], seed=72)
+        fr = p.f.astype(np.complex128).copy()
+        p.adjoint()
+        hg = p.f_hat.astype(np.complex128).copy()
+        nn = p.n
+    finally:
+        p.finalize()
+    tol = 1e-12 if precision == "double" else 1e-5
+    assert rel_errs(fg, O.nfft_trafo(N, nn, m, xr, fhr))[0] < tol
+    assert rel_errs(hg, O.nfft_adjoint(N, nn, m, xr, fr))[0] < tol
