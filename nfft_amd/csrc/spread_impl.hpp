@@ -1234,6 +1234,7 @@ template <typename R> struct BKernelTable {
   fn spread_priv[kMaxDim][kMaxStaticM + 1];  // d = 2, 3 only, 64 threads
   fn spread_rows[kMaxStaticM + 1];            // d = 3 only, 128 threads
   fn spread_runs[kMaxStaticM + 1];            // d = 3, float only, 128 threads
+  fn spread_runs_wide[kMaxStaticM + 1];       // same, runs_wide_nw(W) wavefronts
   fn interp_rt[kMaxDim][2];  // [d-1][tile]
   fn spread_rt[kMaxDim][2];
 };
@@ -1248,8 +1249,10 @@ void bk_fill_one(BKernelTable<R> &k) {
     k.spread_priv[D - 1][M_] = k_spread_wave_priv<R, D, 2 * M_ + 2>;
   }
   if constexpr (D == 3 && D * (2 * M_ + 2) <= 64) k.spread_rows[M_] = k_spread_rows2<R, 2 * M_ + 2, 2>;
-  if constexpr (std::is_same_v<R, float> && D == 3 && D * (2 * M_ + 2) <= 64)
+  if constexpr (std::is_same_v<R, float> && D == 3 && D * (2 * M_ + 2) <= 64) {
     k.spread_runs[M_] = k_spread_runs<2 * M_ + 2, 2, 8>;
+    k.spread_runs_wide[M_] = k_spread_runs<2 * M_ + 2, runs_wide_nw(2 * M_ + 2), 8>;
+  }
 }
 
 template <typename R, int D, int... Ms>
@@ -1292,7 +1295,11 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
   threads = 256;
   if (tile && gg.m >= 1 && gg.m <= kMaxStaticM) {
     const int t = gg.d - 1;
-    if (spread && variant == 4 && gg.d == 3 && k.spread_runs[gg.m]) {
+    if (spread && variant == 5 && gg.d == 3 && k.spread_runs_wide[gg.m]) {
+      threads = 64 * runs_wide_nw(2 * gg.m + 2);
+      return k.spread_runs_wide[gg.m];
+    }
+    if (spread && variant >= 4 && gg.d == 3 && k.spread_runs[gg.m]) {
       threads = 128;
       return k.spread_runs[gg.m];
     }

@@ -35,6 +35,12 @@ __device__ __forceinline__ float rlane_f(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// More wavefronts per tile hide more latency but repeat the per-node window
+// work in each: the widest split with at most five rows groups, NW | W.
+constexpr int runs_wide_nw(int W) {
+  return W % 5 == 0 ? 5 : (W % 4 == 0 ? 4 : (W % 3 == 0 ? 3 : 2));
+}
+
 template <int W, int NW, int G>
 __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restrict__ fin,
                                                          float2 *__restrict__ g,
