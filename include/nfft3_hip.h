@@ -104,7 +104,7 @@ void nfftf_density_pipe_menon_device(nfftf_plan *ths, float *w_dev, int iters);
 
 /* Learned weights: a 3-layer MLP (80 -> 64 -> 64 -> 1, ReLU) on per-node
  * density features (d = 2 plans): the radius 2|x| and, on coarse grids of
- * 16, 32 and 64 cells per dimension, log(1 + count / mean count) of the node
+ * 32, 64 and 128 cells per dimension, log(1 + count / mean count) of the node
  * counts in the 5 x 5 cells around the node (76 values, zero-padded to 80,
  * row-major [M_total][80] floats).  The model predicts log w; its dense
  * layers run on BF16 MFMA with FP32 accumulation.  Parameters are a flat

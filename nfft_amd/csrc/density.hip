@@ -17,7 +17,7 @@
 //    the loss and the optimiser are FP32 vector code.
 //
 //    Features of node j (d = 2), 76 values padded with zeros to 80: the
-//    radius 2|x_j|, then for the coarse grids G = 16, 32, 64 cells per
+//    radius 2|x_j|, then for the coarse grids G = 32, 64, 128 cells per
 //    dimension log(1 + c / cbar) of the node counts c in the 5 x 5 cells
 //    around the node's cell (periodic), cbar = M / G^2.
 #include <cmath>
@@ -38,8 +38,8 @@ constexpr int kDcfFeat = 80;   // padded feature count (multiple of 16)
 constexpr int kDcfRaw = 76;    // 1 + 3 * 25
 constexpr int kDcfHidden = 64;
 constexpr int kDcfScales = 3;
-__constant__ int c_dcf_grid[kDcfScales] = {16, 32, 64};
-constexpr int kDcfGridHost[kDcfScales] = {16, 32, 64};
+__constant__ int c_dcf_grid[kDcfScales] = {32, 64, 128};
+constexpr int kDcfGridHost[kDcfScales] = {32, 64, 128};
 
 // flat FP32 parameter layout; weights [in][out]
 constexpr int P_W1 = 0;

@@ -4,7 +4,7 @@ Restates in numpy what nfft_amd/csrc/density.hip computes, so the GPU
 library can be checked against it:
 
 * ``features``: radius 2|x| and log(1 + c / cbar) of the node counts in the
-  5 x 5 coarse cells around each node at grids of 16, 32, 64 cells.
+  5 x 5 coarse cells around each node at grids of 32, 64, 128 cells.
 * ``pipe_menon``: w <- w / Re(B B^T w) with the oracle's own B and B^T
   (oracle.stage_B / stage_BT, which use the reference's unscaled window;
   the weights agree with the library's after normalisation to mean 1).
@@ -26,7 +26,7 @@ import oracle as O
 NFEAT = 80
 NRAW = 76
 H = 64
-GRIDS = (16, 32, 64)
+GRIDS = (32, 64, 128)
 P_W1 = 0
 P_B1 = P_W1 + NFEAT * H
 P_W2 = P_B1 + H

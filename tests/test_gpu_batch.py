@@ -1,9 +1,11 @@
 """Batched transforms through the C ABI: many right-hand sides on one node
 set (trafo_batch_device / adjoint_batch_device) and several plans in one
 hipGraph (graph_trafo / graph_adjoint).  Both run the same kernels as the
-single calls, so results must be bit-identical to them; graph replays are
-checked one by one, with eager work and synchronisations between replays
-and new input values written before each replay."""
+single calls, so results must equal them: bit for bit where the kernels are
+deterministic (the float family), to 1e-13 relative l2 where B^T adds with
+LDS atomics (the double family's ds_add_f64 order varies from run to run);
+graph replays are checked one by one, with eager work and synchronisations
+between replays and new input values written before each replay."""
 import numpy as np
 import pytest
 
@@ -24,6 +26,13 @@ def plan(N, x, precision="single", m=4):
 
 def cdt(precision):
     return torch.complex64 if precision == "single" else torch.complex128
+
+
+def same(a, b, precision):
+    if precision == "single":
+        assert torch.equal(a, b)
+    else:
+        assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) <= 1e-13
 
 
 def rand_dev(n, seed, precision):
@@ -52,7 +61,7 @@ def test_batch_matches_single_calls(precision):
             p.adjoint_device(fin[r * M:].data_ptr(), oneh.data_ptr())
             p.synchronize()
             assert torch.equal(one, f[r * M:(r + 1) * M])
-            assert torch.equal(oneh, fho[r * NN:(r + 1) * NN])
+            same(oneh, fho[r * NN:(r + 1) * NN], precision)
     finally:
         p.finalize()
 

@@ -34,7 +34,7 @@ def test_features_match_bruteforce():
 def test_features_uniform_density_near_log2():
     # a Cartesian node set: every coarse cell holds the mean count, so every
     # count feature is log(1 + 1)
-    g = (np.arange(64) + 0.5) / 64 - 0.5
+    g = (np.arange(128) + 0.5) / 128 - 0.5
     x = np.stack(np.meshgrid(g, g, indexing="ij"), -1).reshape(-1, 2)
     f = D.features(x)
     assert np.allclose(f[:, 1:D.NRAW], np.log(2.0), atol=1e-12)
