@@ -40,4 +40,10 @@ template <> struct family<double> {
 
 inline long ceil_div(long a, long b) { return (a + b - 1) / b; }
 
+// The HIP runtime's per-thread last-error state also records failures of
+// calls made elsewhere in the process (other libraries, the runtime's own
+// fork handling).  Every entry point clears it first, so that its
+// hipGetLastError() checks report only this library's launches.
+inline void clear_launch_error() { (void)hipGetLastError(); }
+
 }  // namespace nfft_hip

@@ -467,6 +467,7 @@ struct nfft_dcf_model_s {
 extern "C" {
 
 void nfftf_density_pipe_menon_device(nfftf_plan *ths, float *w_dev, int iters) {
+  nfft_hip::clear_launch_error();
   auto *e = nfft_hip::engine_of_mv<float>(ths);
   if (!e) nfft_hip::family<float>::fatal("nfft: density weights need a plan of this library");
   nfft_hip::ensure_nodes_of_mv<float>(ths);
@@ -490,6 +491,7 @@ int nfft_dcf_num_features(void) { return nfft_hip::kDcfFeat; }
 int nfft_dcf_num_params(void) { return nfft_hip::P_COUNT; }
 
 void nfftf_dcf_features_device(nfftf_plan *ths, float *feat_dev) {
+  nfft_hip::clear_launch_error();
   auto *e = nfft_hip::engine_of_mv<float>(ths);
   if (!e) nfft_hip::family<float>::fatal("nfft: density features need a plan of this library");
   nfft_hip::ensure_nodes_of_mv<float>(ths);
@@ -557,6 +559,7 @@ void nfft_dcf_model_set_params(nfft_dcf_model *h, const float *params) {
 
 float nfft_dcf_model_loss_grad(nfft_dcf_model *h, const float *feat_dev, const float *target_dev,
                                int rows, float *grad_host) {
+  nfft_hip::clear_launch_error();
   DcfModel &m = h->m;
   const float loss = m.forward(feat_dev, target_dev, rows, nullptr);
   if (grad_host)
@@ -567,6 +570,7 @@ float nfft_dcf_model_loss_grad(nfft_dcf_model *h, const float *feat_dev, const f
 
 float nfft_dcf_model_train_step(nfft_dcf_model *h, const float *feat_dev, const float *target_dev,
                                 int rows, float lr) {
+  nfft_hip::clear_launch_error();
   using namespace nfft_hip;
   DcfModel &m = h->m;
   const float loss = m.forward(feat_dev, target_dev, rows, nullptr);
@@ -582,6 +586,7 @@ float nfft_dcf_model_train_step(nfft_dcf_model *h, const float *feat_dev, const 
 }
 
 void nfft_dcf_model_predict(nfft_dcf_model *h, const float *feat_dev, int rows, float *out_dev) {
+  nfft_hip::clear_launch_error();
   DcfModel &m = h->m;
   m.forward(feat_dev, nullptr, rows, out_dev);
   NFFT_HIP_CHECK(float, hipStreamSynchronize(m.stream));

@@ -725,6 +725,7 @@ struct PlanApi {
   }
 
   static void precompute(PLAN *p) {
+    clear_launch_error();
     Engine<R> *e = eng(p);
     e->upload_nodes(p->x);
     e->build_nodes();
@@ -753,6 +754,7 @@ struct PlanApi {
   }
 
   static void trafo_direct(const PLAN *p) {
+    clear_launch_error();
     Engine<R> *e = eng(p);
     e->upload_nodes(p->x);
     if (p->N_total > 0)
@@ -766,6 +768,7 @@ struct PlanApi {
   }
 
   static void adjoint_direct(const PLAN *p) {
+    clear_launch_error();
     Engine<R> *e = eng(p);
     e->upload_nodes(p->x);
     if (p->M_total > 0)
@@ -779,6 +782,7 @@ struct PlanApi {
   }
 
   static void trafo(PLAN *p) {
+    clear_launch_error();
     Engine<R> *e = eng(p);
     if (e->direct_only) {
       trafo_direct(p);
@@ -796,6 +800,7 @@ struct PlanApi {
   }
 
   static void adjoint(PLAN *p) {
+    clear_launch_error();
     Engine<R> *e = eng(p);
     if (e->direct_only) {
       adjoint_direct(p);
@@ -813,6 +818,7 @@ struct PlanApi {
   }
 
   static void trafo_device(PLAN *p, const void *fh, void *f) {
+    clear_launch_error();
     Engine<R> *e = eng(p);
     if (e->direct_only) {
       if (!e->nodes_ready) precompute(p);
@@ -825,6 +831,7 @@ struct PlanApi {
   }
 
   static void adjoint_device(PLAN *p, const void *f, void *fh) {
+    clear_launch_error();
     Engine<R> *e = eng(p);
     if (e->direct_only) {
       if (!e->nodes_ready) precompute(p);
@@ -854,6 +861,7 @@ struct PlanApi {
   // allocates, copies from the host or clears memory with memset.
   static nfft_hip_graph *capture(PLAN **plans, int count, const void **in, void **out, int nrhs,
                                  bool adjoint) {
+    clear_launch_error();
     if (count < 1) family<R>::fatal("nfft: a graph needs at least one plan");
     for (int i = 0; i < count; i++) {
       Engine<R> *e = eng(plans[i]);

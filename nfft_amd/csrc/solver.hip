@@ -474,11 +474,13 @@ struct SolverApi {
 #undef NFFT_HIP_CSWAP
 
   static void before_loop(SPLAN *ths) {
+    clear_launch_error();
     if (DevCgnr<R> *dv = find_dev<R>(ths)) dev_before_loop(ths, dv);
     else host_before_loop(ths);
   }
 
   static void loop_one_step(SPLAN *ths) {
+    clear_launch_error();
     if (ths->flags & LANDWEBER) host_landweber(ths);
     if (ths->flags & STEEPEST_DESCENT) host_steepest(ths);
     if (ths->flags & CGNR) {
