@@ -90,8 +90,10 @@ def cpu_baseline(N, n, m, x, fh, f, sample):
 def cgnr_leg(args):
     """CGNR on held-out trajectories: unweighted vs Pipe-Menon vs learned W.
     Two criteria: the metric's normal-equation residual 1e-6 (each solver in
-    its own norm, capped at --cgnr-max-iter steps), and the time to reach the
-    best image error unweighted CGNR attains within 300 steps."""
+    its own norm, capped at --cgnr-max-iter steps), and the time to reach an
+    image error within 5% of the best any of the three attains in 300 steps
+    (with 1% noise the weighted solvers stop improving the image long before
+    their residual reaches 1e-6)."""
     import torch
     from nfft_amd import dcf_train as T
     from nfft_amd import density as D
@@ -131,17 +133,21 @@ def cgnr_leg(args):
                          "s_per_iteration": t_s / max(1, it)}
             log("cgnr %s %s: %d it to residual %.1e (%s), %.1f ms (+%.1f ms weights)"
                 % (desc[0], name, it, r, "reached" if r < tol else "cap", t_s * 1e3, t_w * 1e3))
-        target = min(traces["plain"])
+        # image quality target: within 5% of the best image error any of the
+        # three reaches in 300 steps
+        target = 1.05 * min(min(t) for t in traces.values())
         res["image_target_rel_err"] = target
         for name in ("plain", "pipe_menon", "learned"):
             e = traces[name]
             k = next((i + 1 for i, v in enumerate(e) if v <= target), None)
             spi = res[name]["s_per_iteration"]
-            res[name]["to_plain_best_image"] = {
+            res[name]["to_image_target"] = {
                 "iterations": k, "total_s": None if k is None else k * spi + res[name]["weights_s"],
                 "best_rel_err": min(e)}
-            log("cgnr %s %s: plain's best image (err %.4f) after %s it, own best %.4f"
-                % (desc[0], name, target, k, min(e)))
+            log("cgnr %s %s: image error %.4f after %s it (%s ms), own best %.4f"
+                % (desc[0], name, target, k,
+                   "-" if k is None else "%.1f" % (1e3 * res[name]["to_image_target"]["total_s"]),
+                   min(e)))
         cases.append(res)
         p.finalize()
     model.close()
@@ -156,7 +162,7 @@ def cgnr_leg(args):
         "tol": tol,
         "cases": cases,
     }
-    for crit in ("residual_1e-6", "to_plain_best_image"):
+    for crit in ("residual_1e-6", "to_image_target"):
         t = {k: total(k, crit) for k in ("plain", "pipe_menon", "learned")}
         out[crit] = {"total_s": t,
                      "speedup_learned_vs_plain": (t["plain"] / t["learned"]

@@ -143,7 +143,7 @@ template <int CAP, int RAD> constexpr int stage_slots() { return (CAP / RAD + 25
 // One in-place Stockham stage over `nlines` lines of length n in LDS.
 template <typename R, int RAD, int SIGN, int CAP>
 __device__ __forceinline__ void stage_inplace(cplx<R> *__restrict__ buf, int n, int nlines, int Ns,
-                                              const cplx<R> *__restrict__ tw) {
+                                              const cplx<R> *tw) {
   constexpr int SL = stage_slots<CAP, RAD>();
   const int q = n / RAD;
   const int nbf = nlines * q;
@@ -203,6 +203,9 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
                                                   cplx<R> *__restrict__ fout, FftPass<R> P) {
   extern __shared__ unsigned char fft_smem[];
   cplx<R> *buf = reinterpret_cast<cplx<R> *>(fft_smem);
+  // the line length's twiddle table follows the lines in LDS
+  cplx<R> *tw = buf + (long)P.lines_per_block * P.len;
+  for (int k = threadIdx.x; k < P.len; k += blockDim.x) tw[k] = P.twiddle[k];
   __shared__ long s_goff[kMaxLinesPerBlock];
   __shared__ long s_foff[kMaxLinesPerBlock];
   __shared__ R s_fac[kMaxLinesPerBlock];
@@ -242,54 +245,70 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
   __syncthreads();
 
   // stage the lines: a wave reads along the line for the contiguous axis and
-  // across adjacent lines otherwise
+  // across adjacent lines otherwise.  Each thread issues all its loads
+  // (at most CAP / 256) before the first LDS store, so their latencies
+  // overlap instead of adding up.
   const bool contiguous = P.stride == 1;
-  for (int qd = threadIdx.x; qd < tot; qd += blockDim.x) {
-    int ll, k;
-    if (contiguous) {
-      ll = qd / n;
-      k = qd - ll * n;
-    } else {
-      k = qd / TL;
-      ll = qd - k * TL;
-    }
-    cplx<R> v{R(0), R(0)};
-    if (P.in_mode == kFromGrid) {
-      v = g[s_goff[ll] + (long)k * P.stride];
-    } else {
-      const int ks = band_coord(k, n, P.Nband);
-      if (ks >= 0) {
-        if (P.in_mode == kFromGridBand) {
-          v = g[s_goff[ll] + (long)k * P.stride];
-        } else {
-          const cplx<R> u = fin[s_foff[ll] + (long)ks * P.fstr_ax];
-          const R f = s_fac[ll] * P.phi_ax[ks];
-          v = cplx<R>{u.x * f, u.y * f};
+  constexpr int PER = CAP / 256;
+  cplx<R> v[PER];
+  int dst[PER];
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int qd = threadIdx.x + u * 256;
+    v[u] = cplx<R>{R(0), R(0)};
+    dst[u] = -1;
+    if (qd < tot) {
+      int ll, k;
+      if (contiguous) {
+        ll = qd / n;
+        k = qd - ll * n;
+      } else {
+        k = qd / TL;
+        ll = qd - k * TL;
+      }
+      dst[u] = ll * n + k;
+      if (P.in_mode == kFromGrid) {
+        v[u] = g[s_goff[ll] + (long)k * P.stride];
+      } else {
+        const int ks = band_coord(k, n, P.Nband);
+        if (ks >= 0) {
+          if (P.in_mode == kFromGridBand) {
+            v[u] = g[s_goff[ll] + (long)k * P.stride];
+          } else {
+            const cplx<R> w = fin[s_foff[ll] + (long)ks * P.fstr_ax];
+            const R f = s_fac[ll] * P.phi_ax[ks];
+            v[u] = cplx<R>{w.x * f, w.y * f};
+          }
         }
       }
     }
-    buf[ll * n + k] = v;
   }
+#pragma unroll
+  for (int u = 0; u < PER; u++)
+    if (dst[u] >= 0) buf[dst[u]] = v[u];
   __syncthreads();
 
   int Ns = 1;
   for (int st = 0; st < P.nstages; st++) {
     const int rad = P.radix[st];
     switch (rad) {
-      case 2: stage_inplace<R, 2, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
-      case 3: stage_inplace<R, 3, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
-      case 4: stage_inplace<R, 4, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
-      case 5: stage_inplace<R, 5, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
-      case 7: stage_inplace<R, 7, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
-      case 8: stage_inplace<R, 8, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
-      case 11: stage_inplace<R, 11, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
-      case 13: stage_inplace<R, 13, SIGN, CAP>(buf, n, TL, Ns, P.twiddle); break;
+      case 2: stage_inplace<R, 2, SIGN, CAP>(buf, n, TL, Ns, tw); break;
+      case 3: stage_inplace<R, 3, SIGN, CAP>(buf, n, TL, Ns, tw); break;
+      case 4: stage_inplace<R, 4, SIGN, CAP>(buf, n, TL, Ns, tw); break;
+      case 5: stage_inplace<R, 5, SIGN, CAP>(buf, n, TL, Ns, tw); break;
+      case 7: stage_inplace<R, 7, SIGN, CAP>(buf, n, TL, Ns, tw); break;
+      case 8: stage_inplace<R, 8, SIGN, CAP>(buf, n, TL, Ns, tw); break;
+      case 11: stage_inplace<R, 11, SIGN, CAP>(buf, n, TL, Ns, tw); break;
+      case 13: stage_inplace<R, 13, SIGN, CAP>(buf, n, TL, Ns, tw); break;
       default: break;  // excluded at plan time
     }
     Ns *= rad;
   }
 
-  for (int qd = threadIdx.x; qd < tot; qd += blockDim.x) {
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int qd = threadIdx.x + u * 256;
+    if (qd >= tot) break;
     int ll, k;
     if (contiguous) {
       ll = qd / n;
@@ -298,17 +317,17 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
       k = qd / TL;
       ll = qd - k * TL;
     }
-    const cplx<R> v = buf[ll * n + k];
+    const cplx<R> w = buf[ll * n + k];
     if (P.out_mode == kToGrid) {
-      g[s_goff[ll] + (long)k * P.stride] = v;
+      g[s_goff[ll] + (long)k * P.stride] = w;
     } else {
       const int ks = band_coord(k, n, P.Nband);
       if (ks >= 0) {
         if (P.out_mode == kToGridBand) {
-          g[s_goff[ll] + (long)k * P.stride] = v;
+          g[s_goff[ll] + (long)k * P.stride] = w;
         } else {
           const R f = s_fac[ll] * P.phi_ax[ks];
-          fout[s_foff[ll] + (long)ks * P.fstr_ax] = cplx<R>{v.x * f, v.y * f};
+          fout[s_foff[ll] + (long)ks * P.fstr_ax] = cplx<R>{w.x * f, w.y * f};
         }
       }
     }
@@ -323,7 +342,7 @@ void launch_fft_pass(cplx<R> *g, const cplx<R> *fhat_in, cplx<R> *fhat_out, cons
                      int sign, hipStream_t s) {
   if (p.nlines <= 0) return;
   const long blocks = ceil_div(p.nlines, p.lines_per_block);
-  const size_t lds = sizeof(cplx<R>) * (size_t)p.lines_per_block * (size_t)p.len;
+  const size_t lds = sizeof(cplx<R>) * ((size_t)p.lines_per_block + 1) * (size_t)p.len;
   constexpr int CAP = fft_cap<R>();
   if (sign < 0)
     hipLaunchKernelGGL((k_fft_pass<R, -1, CAP>), dim3((unsigned)blocks), dim3(256), lds, s, g,

@@ -178,11 +178,15 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     win.poly = poly;
   }
 
-  // bins: shrink the largest edge until the LDS tile fits the budget
+  // bins: start from an edge that gives the grid enough bins to fill the
+  // GPU (1-D 256 cells, 2-D n/32 clamped to [16, 64], 3-D 16), then shrink
+  // the largest edge until the LDS tile fits the budget
   const size_t budget = 48 * 1024;
   int bs[kMaxDim] = {1, 1, 1};
-  const int init = d == 1 ? 4096 : d == 2 ? 64 : 16;
-  for (int t = 0; t < d; t++) bs[t] = (int)std::min<long>(n[t], init);
+  for (int t = 0; t < d; t++) {
+    const long init = d == 1 ? 256 : d == 2 ? std::min<long>(64, std::max<long>(16, n[t] / 32)) : 16;
+    bs[t] = (int)std::min<long>(n[t], init);
+  }
   auto bytes = [&]() {
     size_t s = sizeof(cplx<R>);
     for (int t = 0; t < d; t++) s *= (size_t)(bs[t] + 2 * m + 1);
@@ -205,7 +209,7 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     geom.E[t] = t < d ? bs[t] + 2 * m + 1 : 1;
   }
   max_per_sub = d == 3 ? 1024 : 512;
-  spread_max_per_sub = d == 3 ? 2048 : 4096;
+  spread_max_per_sub = d == 3 ? 2048 : 1024;
   // B^T writes its LDS tiles to scratch and reduces them without atomics
   // when the tile fits LDS and never wraps onto itself (E_t <= n_t)
   {

@@ -247,13 +247,27 @@ __device__ __forceinline__ void load_region_3d(const cplx<R> *__restrict__ g, cp
   if (r < rows_per_pass) {
     const long col = wrap(lo[2] - gg.m + e2, gg.n[2]);
     int e0 = r / E1, e1 = r - (r / E1) * E1;
-    for (; e0 < E0;) {
-      tile[(e0 * E1 + e1) * E2 + e2] = g[rowoff0[e0] + rowoff1[e1] + col];
-      e1 += rows_per_pass;
-      while (e1 >= E1) {
-        e1 -= E1;
-        e0++;
+    // eight loads in flight before their LDS stores
+    constexpr int B = 8;
+    while (e0 < E0) {
+      cplx<R> v[B];
+      int dst[B];
+#pragma unroll
+      for (int u = 0; u < B; u++) {
+        dst[u] = -1;
+        if (e0 < E0) {
+          dst[u] = (e0 * E1 + e1) * E2 + e2;
+          v[u] = g[rowoff0[e0] + rowoff1[e1] + col];
+          e1 += rows_per_pass;
+          while (e1 >= E1) {
+            e1 -= E1;
+            e0++;
+          }
+        }
       }
+#pragma unroll
+      for (int u = 0; u < B; u++)
+        if (dst[u] >= 0) tile[dst[u]] = v[u];
     }
   }
 }
@@ -1087,18 +1101,22 @@ __global__ __launch_bounds__(256) void k_gather_tiles(const cplx<R> *__restrict_
 // dimension, the right zone of its left neighbour (core cells [0, m+1)),
 // its own core zone, and the left zone of its right neighbour (core cells
 // [bs-m, bs)); every work item of each of the 3^d neighbour bins contributes
-// one contiguous zone.  Threads own fixed core cells and sum in registers;
-// each grid cell is written once.
-constexpr int kZoneCellsPerThread = 16;
-template <typename R, int D>
+// one contiguous zone.  The (zone, work item) pairs are listed in LDS; each
+// thread owns CPT core cells, sums in registers and keeps up to 16 loads in
+// flight; each grid cell is written once.
+constexpr int kZonePairs = 512;  // pairs listed per pass
+template <typename R, int D, int CPT>
 __global__ __launch_bounds__(256) void k_gather_zones(const cplx<R> *__restrict__ tiles,
                                                       const int *__restrict__ sp_first,
                                                       const int *__restrict__ sp_count,
                                                       cplx<R> *__restrict__ g, BinGeom gg) {
   constexpr int NZ = D == 1 ? 3 : (D == 2 ? 9 : 27);
-  __shared__ long s_base[NZ];   // tile offset of the zone in item 0 of the neighbour
-  __shared__ int s_first[NZ], s_cnt[NZ];
+  constexpr int BATCH = CPT >= 16 ? 1 : 16 / CPT;
+  __shared__ long s_base[NZ];   // offset of the zone inside a tile
+  __shared__ int s_first[NZ], s_cnt[NZ], s_start[NZ + 1];
   __shared__ int s_off[NZ][3], s_size[NZ][3];
+  __shared__ long s_pbase[kZonePairs];
+  __shared__ int s_pz[kZonePairs];
   int bc[3] = {0, 0, 0};
   {
     int r = blockIdx.x;
@@ -1121,17 +1139,16 @@ __global__ __launch_bounds__(256) void k_gather_zones(const cplx<R> *__restrict_
       r /= 3;
     }
     int nbl = 0, zidx = 0;
-    int zsz[3] = {1, 1, 1};
 #pragma unroll
     for (int t = 0; t < D; t++) {
       const int nb = (bc[t] + dig[t] - 1 + gg.nb[t]) % gg.nb[t];
       nbl = nbl * gg.nb[t] + nb;
       const int z = 2 - dig[t];
-      int st;
-      zone_dims<D>(gg, t, z, st, zsz[t]);
+      int st, sz;
+      zone_dims<D>(gg, t, z, st, sz);
       zidx = zidx * 3 + z;
-      s_size[threadIdx.x][t] = zsz[t];
-      s_off[threadIdx.x][t] = dig[t] == 0 ? 0 : (dig[t] == 1 ? 0 : gg.bs[t] - gg.m);
+      s_size[threadIdx.x][t] = sz;
+      s_off[threadIdx.x][t] = dig[t] == 2 ? gg.bs[t] - gg.m : 0;
     }
     // offset of zone zidx: sum of the sizes of the zones before it
     long base = 0;
@@ -1150,13 +1167,23 @@ __global__ __launch_bounds__(256) void k_gather_zones(const cplx<R> *__restrict_
     s_cnt[threadIdx.x] = sp_count[nbl];
   }
   __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int z = 0; z < NZ; z++) {
+      s_start[z] = acc;
+      acc += s_cnt[z];
+    }
+    s_start[NZ] = acc;
+  }
+  __syncthreads();
+  const int npairs = s_start[NZ];
   int core = 1;
 #pragma unroll
   for (int t = 0; t < D; t++) core *= gg.bs[t];
-  int kc[kZoneCellsPerThread][3];
-  R ar[kZoneCellsPerThread], ai[kZoneCellsPerThread];
+  int kc[CPT][3];
+  R ar[CPT], ai[CPT];
 #pragma unroll
-  for (int i = 0; i < kZoneCellsPerThread; i++) {
+  for (int i = 0; i < CPT; i++) {
     const int q = threadIdx.x + i * blockDim.x;
     int rem = q < core ? q : 0;
     kc[i][0] = kc[i][1] = kc[i][2] = 0;
@@ -1167,38 +1194,55 @@ __global__ __launch_bounds__(256) void k_gather_zones(const cplx<R> *__restrict_
     }
     ar[i] = ai[i] = R(0);
   }
-  for (int z = 0; z < NZ; z++) {
-    const int cnt = s_cnt[z];
-    if (cnt == 0) continue;
-    const cplx<R> *zb = tiles + (long)s_first[z] * tot + s_base[z];
-#pragma unroll
-    for (int i = 0; i < kZoneCellsPerThread; i++) {
-      const int q = threadIdx.x + i * blockDim.x;
-      if (q >= core) break;
-      bool in = true;
-      int idx = 0;
-#pragma unroll
-      for (int t = 0; t < D; t++) {
-        const int u = kc[i][t] - s_off[z][t];
-        in = in && u >= 0 && u < s_size[z][t];
-        idx = idx * s_size[z][t] + u;
-      }
-      if (!in) continue;
-      for (int c = 0; c < cnt; c++) {
-        const cplx<R> v = zb[(long)c * tot + idx];
-        ar[i] += v.x;
-        ai[i] += v.y;
-      }
+  for (int w0 = 0; w0 < npairs; w0 += kZonePairs) {
+    const int nw = min(kZonePairs, npairs - w0);
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) {
+      const int pi = w0 + i;
+      int z = 0;
+      while (s_start[z + 1] <= pi) z++;
+      s_pz[i] = z;
+      s_pbase[i] = (long)(s_first[z] + pi - s_start[z]) * tot + s_base[z];
     }
+    __syncthreads();
+    for (int i0 = 0; i0 < nw; i0 += BATCH) {
+      cplx<R> v[BATCH][CPT];
+#pragma unroll
+      for (int u = 0; u < BATCH; u++) {
+        const int i = i0 + u;
+        const int z = i < nw ? s_pz[i] : 0;
+        const long pb = i < nw ? s_pbase[i] : 0;
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+          const int q = threadIdx.x + c * blockDim.x;
+          bool in = i < nw && q < core;
+          int idx = 0;
+#pragma unroll
+          for (int t = 0; t < D; t++) {
+            const int uu = kc[c][t] - s_off[z][t];
+            in = in && uu >= 0 && uu < s_size[z][t];
+            idx = idx * s_size[z][t] + uu;
+          }
+          v[u][c] = in ? tiles[pb + idx] : cplx<R>{R(0), R(0)};
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < BATCH; u++)
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+          ar[c] += v[u][c].x;
+          ai[c] += v[u][c].y;
+        }
+    }
+    __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < kZoneCellsPerThread; i++) {
-    const int q = threadIdx.x + i * blockDim.x;
+  for (int c = 0; c < CPT; c++) {
+    const int q = threadIdx.x + c * blockDim.x;
     if (q >= core) break;
     long gi = 0;
 #pragma unroll
-    for (int t = 0; t < D; t++) gi = gi * gg.n[t] + bc[t] * gg.bs[t] + kc[i][t];
-    g[gi] = cplx<R>{ar[i], ai[i]};
+    for (int t = 0; t < D; t++) gi = gi * gg.n[t] + bc[t] * gg.bs[t] + kc[c][t];
+    g[gi] = cplx<R>{ar[c], ai[c]};
   }
 }
 
@@ -1209,9 +1253,16 @@ void launch_gather_tiles(const Engine<R> &e, cplx<R> *g, hipStream_t s) {
   if (e.geom.zoned) {
     long core = 1;
     for (int t = 0; t < e.geom.d; t++) core *= e.geom.bs[t];
-    if (core <= 256L * kZoneCellsPerThread) {
-      auto fz = e.geom.d == 1 ? k_gather_zones<R, 1> : e.geom.d == 2 ? k_gather_zones<R, 2>
-                                                                     : k_gather_zones<R, 3>;
+    if (core <= 4096) {
+      using F = void (*)(const cplx<R> *, const int *, const int *, cplx<R> *, BinGeom);
+      F fz;
+      if (e.geom.d == 3)
+        fz = core <= 512 ? k_gather_zones<R, 3, 2> : (core <= 1024 ? k_gather_zones<R, 3, 4>
+                                                                    : k_gather_zones<R, 3, 16>);
+      else if (e.geom.d == 2)
+        fz = core <= 1024 ? k_gather_zones<R, 2, 4> : k_gather_zones<R, 2, 16>;
+      else
+        fz = core <= 1024 ? k_gather_zones<R, 1, 4> : k_gather_zones<R, 1, 16>;
       hipLaunchKernelGGL(fz, dim3((unsigned)nbins), dim3(256), 0, s, e.tiles, e.sp_first,
                          e.sp_count, g, e.geom);
       return;
