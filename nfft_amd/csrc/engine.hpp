@@ -64,6 +64,10 @@ template <typename R> struct FftAxis {
 enum FftIn { kFromGrid = 0, kFromGridBand = 1, kFromCoeffs = 2 };
 enum FftOut { kToGrid = 0, kToGridBand = 1, kToCoeffs = 2 };
 
+// LDS line stride (complex elements) of the FFT passes: odd, with one pad
+// element per 16 (fft.hip)
+__host__ __device__ inline int lds_line_stride(int n) { return n + (n >> 4) + 1; }
+
 // One pruned pass of the oversampled FFT along dimension t (fft.hip): the
 // lines are all combinations of the other dimensions' coordinates, each
 // either over the full grid or over the band only (dimensions whose band is
@@ -87,6 +91,7 @@ template <typename R> struct FftPass {
   int nstages;
   int radix[20];
   const cplx<R> *twiddle;
+  int twiddle_in_lds;  // copy the twiddle table to LDS (when it fits 64 KB)
 };
 
 template <typename R> struct Engine {

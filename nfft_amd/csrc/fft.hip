@@ -140,28 +140,38 @@ __device__ __forceinline__ void dft(cplx<R> *v) {
 // Butterflies per thread of one stage: lines * n / RAD over the workgroup.
 template <int CAP, int RAD> constexpr int stage_slots() { return (CAP / RAD + 255) / 256; }
 
-// One in-place Stockham stage over `nlines` lines of length n in LDS.
-template <typename R, int RAD, int SIGN, int CAP>
-__device__ __forceinline__ void stage_inplace(cplx<R> *__restrict__ buf, int n, int nlines, int Ns,
-                                              const cplx<R> *tw) {
+// LDS layout of the staged lines: element k of line ll sits at
+// ll * ls + k + (k >> 4) with an odd line stride ls = n + n/16 + 1, so that
+// neither the strided accesses of the first Stockham stages (stride RAD
+// within a line) nor the staging of adjacent lines (one element per line)
+// fall into the same LDS bank.
+__device__ __forceinline__ int lds_idx(int ll, int k, int ls) { return ll * ls + k + (k >> 4); }
+
+// One in-place Stockham stage over `nlines` lines of length n in LDS.  POW2:
+// n and Ns are powers of two (shifts instead of divisions).
+template <typename R, int RAD, int SIGN, int CAP, bool POW2>
+__device__ __forceinline__ void stage_inplace(cplx<R> *__restrict__ buf, int n, int ls, int nlines,
+                                              int Ns, const cplx<R> *tw) {
   constexpr int SL = stage_slots<CAP, RAD>();
   const int q = n / RAD;
   const int nbf = nlines * q;
   const int span = Ns * RAD;
   const int twstep = n / span;
+  const int lq = POW2 ? __builtin_ctz(q) : 0;
+  const int lns = POW2 ? __builtin_ctz(Ns) : 0;
   cplx<R> v[SL][RAD];
-  int dsto[SL];
+  int dll[SL], dk[SL];
 #pragma unroll
   for (int i = 0; i < SL; i++) {
     const int b = threadIdx.x + i * 256;
-    dsto[i] = -1;
+    dll[i] = -1;
     if (b < nbf) {
-      const int ll = b / q;
+      const int ll = POW2 ? b >> lq : b / q;
       const int j = b - ll * q;
-      const int jm = j % Ns;
-      const cplx<R> *src = buf + ll * n + j;
+      const int jm = POW2 ? j & (Ns - 1) : j % Ns;
+      const int jd = POW2 ? j >> lns : j / Ns;
 #pragma unroll
-      for (int r = 0; r < RAD; r++) v[i][r] = src[r * q];
+      for (int r = 0; r < RAD; r++) v[i][r] = buf[lds_idx(ll, j + r * q, ls)];
       if (Ns > 1) {
 #pragma unroll
         for (int r = 1; r < RAD; r++) {
@@ -171,16 +181,16 @@ __device__ __forceinline__ void stage_inplace(cplx<R> *__restrict__ buf, int n, 
         }
       }
       dft<R, RAD, SIGN>(v[i]);
-      dsto[i] = ll * n + (j / Ns) * span + jm;
+      dll[i] = ll;
+      dk[i] = jd * span + jm;
     }
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < SL; i++) {
-    if (dsto[i] >= 0) {
-      cplx<R> *o = buf + dsto[i];
+    if (dll[i] >= 0) {
 #pragma unroll
-      for (int r = 0; r < RAD; r++) o[r * Ns] = v[i][r];
+      for (int r = 0; r < RAD; r++) buf[lds_idx(dll[i], dk[i] + r * Ns, ls)] = v[i][r];
     }
   }
   __syncthreads();
@@ -197,15 +207,20 @@ __device__ __forceinline__ int band_coord(int k, int n, int N) {
 
 constexpr int kMaxLinesPerBlock = 256;
 
-template <typename R, int SIGN, int CAP>
+template <typename R, int SIGN, int CAP, bool POW2>
 __global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
                                                   const cplx<R> *__restrict__ fin,
                                                   cplx<R> *__restrict__ fout, FftPass<R> P) {
   extern __shared__ unsigned char fft_smem[];
   cplx<R> *buf = reinterpret_cast<cplx<R> *>(fft_smem);
-  // the line length's twiddle table follows the lines in LDS
-  cplx<R> *tw = buf + (long)P.lines_per_block * P.len;
-  for (int k = threadIdx.x; k < P.len; k += blockDim.x) tw[k] = P.twiddle[k];
+  const int ls = lds_line_stride(P.len);
+  // the line length's twiddle table follows the lines in LDS when it fits
+  const cplx<R> *tw = P.twiddle;
+  if (P.twiddle_in_lds) {
+    cplx<R> *twl = buf + (long)P.lines_per_block * ls;
+    for (int k = threadIdx.x; k < P.len; k += blockDim.x) twl[k] = P.twiddle[k];
+    tw = twl;
+  }
   __shared__ long s_goff[kMaxLinesPerBlock];
   __shared__ long s_foff[kMaxLinesPerBlock];
   __shared__ R s_fac[kMaxLinesPerBlock];
@@ -266,7 +281,7 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
         k = qd / TL;
         ll = qd - k * TL;
       }
-      dst[u] = ll * n + k;
+      dst[u] = lds_idx(ll, k, ls);
       if (P.in_mode == kFromGrid) {
         v[u] = g[s_goff[ll] + (long)k * P.stride];
       } else {
@@ -291,16 +306,24 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
   int Ns = 1;
   for (int st = 0; st < P.nstages; st++) {
     const int rad = P.radix[st];
-    switch (rad) {
-      case 2: stage_inplace<R, 2, SIGN, CAP>(buf, n, TL, Ns, tw); break;
-      case 3: stage_inplace<R, 3, SIGN, CAP>(buf, n, TL, Ns, tw); break;
-      case 4: stage_inplace<R, 4, SIGN, CAP>(buf, n, TL, Ns, tw); break;
-      case 5: stage_inplace<R, 5, SIGN, CAP>(buf, n, TL, Ns, tw); break;
-      case 7: stage_inplace<R, 7, SIGN, CAP>(buf, n, TL, Ns, tw); break;
-      case 8: stage_inplace<R, 8, SIGN, CAP>(buf, n, TL, Ns, tw); break;
-      case 11: stage_inplace<R, 11, SIGN, CAP>(buf, n, TL, Ns, tw); break;
-      case 13: stage_inplace<R, 13, SIGN, CAP>(buf, n, TL, Ns, tw); break;
-      default: break;  // excluded at plan time
+    if constexpr (POW2) {
+      switch (rad) {
+        case 2: stage_inplace<R, 2, SIGN, CAP, true>(buf, n, ls, TL, Ns, tw); break;
+        case 4: stage_inplace<R, 4, SIGN, CAP, true>(buf, n, ls, TL, Ns, tw); break;
+        default: stage_inplace<R, 8, SIGN, CAP, true>(buf, n, ls, TL, Ns, tw); break;
+      }
+    } else {
+      switch (rad) {
+        case 2: stage_inplace<R, 2, SIGN, CAP, false>(buf, n, ls, TL, Ns, tw); break;
+        case 3: stage_inplace<R, 3, SIGN, CAP, false>(buf, n, ls, TL, Ns, tw); break;
+        case 4: stage_inplace<R, 4, SIGN, CAP, false>(buf, n, ls, TL, Ns, tw); break;
+        case 5: stage_inplace<R, 5, SIGN, CAP, false>(buf, n, ls, TL, Ns, tw); break;
+        case 7: stage_inplace<R, 7, SIGN, CAP, false>(buf, n, ls, TL, Ns, tw); break;
+        case 8: stage_inplace<R, 8, SIGN, CAP, false>(buf, n, ls, TL, Ns, tw); break;
+        case 11: stage_inplace<R, 11, SIGN, CAP, false>(buf, n, ls, TL, Ns, tw); break;
+        case 13: stage_inplace<R, 13, SIGN, CAP, false>(buf, n, ls, TL, Ns, tw); break;
+        default: break;  // excluded at plan time
+      }
     }
     Ns *= rad;
   }
@@ -317,7 +340,7 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
       k = qd / TL;
       ll = qd - k * TL;
     }
-    const cplx<R> w = buf[ll * n + k];
+    const cplx<R> w = buf[lds_idx(ll, k, ls)];
     if (P.out_mode == kToGrid) {
       g[s_goff[ll] + (long)k * P.stride] = w;
     } else {
@@ -342,14 +365,14 @@ void launch_fft_pass(cplx<R> *g, const cplx<R> *fhat_in, cplx<R> *fhat_out, cons
                      int sign, hipStream_t s) {
   if (p.nlines <= 0) return;
   const long blocks = ceil_div(p.nlines, p.lines_per_block);
-  const size_t lds = sizeof(cplx<R>) * ((size_t)p.lines_per_block + 1) * (size_t)p.len;
+  const size_t lds = sizeof(cplx<R>) * ((size_t)p.lines_per_block * (size_t)lds_line_stride(p.len) +
+                                        (p.twiddle_in_lds ? (size_t)p.len : 0));
   constexpr int CAP = fft_cap<R>();
-  if (sign < 0)
-    hipLaunchKernelGGL((k_fft_pass<R, -1, CAP>), dim3((unsigned)blocks), dim3(256), lds, s, g,
-                       fhat_in, fhat_out, p);
-  else
-    hipLaunchKernelGGL((k_fft_pass<R, +1, CAP>), dim3((unsigned)blocks), dim3(256), lds, s, g,
-                       fhat_in, fhat_out, p);
+  bool pow2 = true;
+  for (int i = 0; i < p.nstages; i++) pow2 = pow2 && (p.radix[i] == 2 || p.radix[i] == 4 || p.radix[i] == 8);
+  auto fn = sign < 0 ? (pow2 ? k_fft_pass<R, -1, CAP, true> : k_fft_pass<R, -1, CAP, false>)
+                     : (pow2 ? k_fft_pass<R, +1, CAP, true> : k_fft_pass<R, +1, CAP, false>);
+  hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(256), lds, s, g, fhat_in, fhat_out, p);
 }
 
 template void launch_fft_pass<float>(cplx<float> *, const cplx<float> *, cplx<float> *,

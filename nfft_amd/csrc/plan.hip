@@ -314,6 +314,8 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
       p.nstages = axes[t].nstages;
       for (int i = 0; i < p.nstages; i++) p.radix[i] = axes[t].radix[i];
       p.twiddle = axes[t].twiddle;
+      const size_t line_bytes = sizeof(cplx<R>) * (size_t)p.lines_per_block * lds_line_stride(p.len);
+      p.twiddle_in_lds = line_bytes + sizeof(cplx<R>) * (size_t)p.len <= 64 * 1024 ? 1 : 0;
       return p;
     };
     for (int t = d - 1; t >= 0; t--)
