@@ -265,9 +265,19 @@ def main():
     b_trafo_ms = stages_ms["trafo_B"]
     b_adj_ms = stages_ms["adjoint_B"]
     if b_adj_ms >= b_trafo_ms:
-        dom_name, dom_ms = "B^T spreading (k_spread_runs, adjoint)", b_adj_ms
+        dom_name, dom_ms, dom_sym = "B^T spreading (k_spread_runs, adjoint)", b_adj_ms, "k_spread_runs"
     else:
-        dom_name, dom_ms = "B interpolation (k_interp_tile, trafo)", b_trafo_ms
+        dom_name, dom_ms, dom_sym = "B interpolation (k_interp_tile, trafo)", b_trafo_ms, "k_interp_tile"
+    # HBM bytes per launch of that kernel measured by rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE (copy-calibrated) and committed with the profile summaries
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_kooshball3d.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh_:
+            tj = json.load(fh_)
+        for k, v in tj.get("kernels", {}).items():
+            if dom_sym in k:
+                traffic = v["traffic_bytes"]
     achieved = ab["B"] / (dom_ms * 1e-3) / 1e9
     step_bytes = ab["trafo"] + ab["adjoint"]
 
@@ -302,7 +312,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "profiles/traffic_kooshball3d.json (rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                              "per dispatch, scaled by a 1 GiB copy kernel's readings)",
             "avg_launch_ms": dom_ms,
             "algorithmic_bytes_per_launch": ab["B"],
         },
