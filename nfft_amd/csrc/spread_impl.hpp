@@ -1286,6 +1286,7 @@ template <typename R> struct BKernelTable {
   fn spread_rows[kMaxStaticM + 1];            // d = 3 only, 128 threads
   fn spread_runs[kMaxStaticM + 1];            // d = 3, float only, 128 threads
   fn spread_runs_wide[kMaxStaticM + 1];       // same, runs_wide_nw(W) wavefronts
+  fn spread_runs_2s[kMaxStaticM + 1];         // same, two node streams per wavefront
   fn interp_rt[kMaxDim][2];  // [d-1][tile]
   fn spread_rt[kMaxDim][2];
 };
@@ -1301,8 +1302,9 @@ void bk_fill_one(BKernelTable<R> &k) {
   }
   if constexpr (D == 3 && D * (2 * M_ + 2) <= 64) k.spread_rows[M_] = k_spread_rows2<R, 2 * M_ + 2, 2>;
   if constexpr (std::is_same_v<R, float> && D == 3 && D * (2 * M_ + 2) <= 64) {
-    k.spread_runs[M_] = k_spread_runs<2 * M_ + 2, 2, 8>;
-    k.spread_runs_wide[M_] = k_spread_runs<2 * M_ + 2, runs_wide_nw(2 * M_ + 2), 8>;
+    k.spread_runs[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 1>;
+    k.spread_runs_wide[M_] = k_spread_runs<2 * M_ + 2, runs_wide_nw(2 * M_ + 2), 8, 1>;
+    k.spread_runs_2s[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2>;
   }
 }
 
@@ -1346,6 +1348,10 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
   threads = 256;
   if (tile && gg.m >= 1 && gg.m <= kMaxStaticM) {
     const int t = gg.d - 1;
+    if (spread && variant == 6 && gg.d == 3 && k.spread_runs_2s[gg.m]) {
+      threads = 128;
+      return k.spread_runs_2s[gg.m];
+    }
     if (spread && variant == 5 && gg.d == 3 && k.spread_runs_wide[gg.m]) {
       threads = 64 * runs_wide_nw(2 * gg.m + 2);
       return k.spread_runs_wide[gg.m];

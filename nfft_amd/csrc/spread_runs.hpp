@@ -41,7 +41,12 @@ constexpr int runs_wide_nw(int W) {
   return W % 5 == 0 ? 5 : (W % 4 == 0 ? 4 : (W % 3 == 0 ? 3 : 2));
 }
 
-template <int W, int NW, int G>
+// S node streams per wavefront: stream s walks its own part of the work
+// item's nodes and the updates of the streams interleave node by node, which
+// gives the wavefront S independent dependency chains.  Streams of one
+// wavefront may touch the same tile cells; a wavefront's LDS operations
+// complete in program order, so their read-modify-writes do not race.
+template <int W, int NW, int G, int S>
 __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restrict__ fin,
                                                          float2 *__restrict__ g,
                                                          const int *__restrict__ perm,
@@ -76,17 +81,17 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
   bool pv[SLOTS];
   int adr_u[NP][SLOTS], adr_v0[NP][SLOTS], adr_v1[NP][SLOTS];
 #pragma unroll
-  for (int s = 0; s < SLOTS; s++) {
-    const int q = lane + 64 * s;
-    pv[s] = q < PAIRS;
-    const int pa = pv[s] ? q / HW : 0;
-    const int pb = pv[s] ? 2 * (q - (q / HW) * HW) : 0;
-    off[s] = pa * E2 + pb;
+  for (int sl = 0; sl < SLOTS; sl++) {
+    const int q = lane + 64 * sl;
+    pv[sl] = q < PAIRS;
+    const int pa = pv[sl] ? q / HW : 0;
+    const int pb = pv[sl] ? 2 * (q - (q / HW) * HW) : 0;
+    off[sl] = pa * E2 + pb;
 #pragma unroll
     for (int h = 0; h < NP; h++) {
-      adr_u[h][s] = 4 * (h * 32 + W + pa);
-      adr_v0[h][s] = 4 * (h * 32 + 2 * W + pb);
-      adr_v1[h][s] = 4 * (h * 32 + 2 * W + pb + 1);
+      adr_u[h][sl] = 4 * (h * 32 + W + pa);
+      adr_v0[h][sl] = 4 * (h * 32 + 2 * W + pb);
+      adr_v1[h][sl] = 4 * (h * 32 + 2 * W + pb + 1);
     }
   }
   // window tap evaluated by this lane in a pass: node half h, dimension wt, tap ww
@@ -101,144 +106,187 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
     for (int k = 0; k < kPolyCoef; k++) coef[k] = pc[k];
   }
 
-  // run accumulators: [row j][slot][pair cell], complex
-  float2 acc[RW][SLOTS][2];
+  // streams: node ranges [sb, sb + sl) of the work item
+  int sb[S], sn[S];
+  {
+    const int part = (sp.z + S - 1) / S;
 #pragma unroll
-  for (int j = 0; j < RW; j++)
+    for (int st = 0; st < S; st++) {
+      sb[st] = min(sp.z, st * part);
+      sn[st] = min(sp.z, (st + 1) * part) - sb[st];
+    }
+  }
+  // run accumulators per stream: [row j][slot][pair cell], complex
+  float2 acc[S][RW][SLOTS][2];
 #pragma unroll
-    for (int s = 0; s < SLOTS; s++) acc[j][s][0] = acc[j][s][1] = float2{0.f, 0.f};
-  // current run: packed tile corner | (row a0 << 20); -1 when none
-  int cur = -1;
-  auto flush = [&]() {
-    const int corner = cur & 0xfffff;
-    const int a0 = cur >> 20;
+  for (int st = 0; st < S; st++)
+#pragma unroll
+    for (int j = 0; j < RW; j++)
+#pragma unroll
+      for (int sl = 0; sl < SLOTS; sl++) acc[st][j][sl][0] = acc[st][j][sl][1] = float2{0.f, 0.f};
+  // current run per stream: packed tile corner | (row a0 << 20); -1 when none
+  int cur[S];
+#pragma unroll
+  for (int st = 0; st < S; st++) cur[st] = -1;
+  auto flush = [&](int st) {
+    const int corner = cur[st] & 0xfffff;
+    const int a0 = cur[st] >> 20;
     const int r0 = ((wv - a0) % NW + NW) % NW;
     // all reads, then all writes: the rows are distinct cells
 #pragma unroll
-    for (int s = 0; s < SLOTS; s++) {
-      if (pv[s]) {
+    for (int sl = 0; sl < SLOTS; sl++) {
+      if (pv[sl]) {
         float2 v[RW][2];
 #pragma unroll
         for (int j = 0; j < RW; j++) {
-          const float2 *t = tile + corner + (r0 + j * NW) * plane + off[s];
+          const float2 *t = tile + corner + (r0 + j * NW) * plane + off[sl];
           v[j][0] = t[0];
           v[j][1] = t[1];
         }
 #pragma unroll
         for (int j = 0; j < RW; j++) {
-          float2 *t = tile + corner + (r0 + j * NW) * plane + off[s];
-          t[0] = float2{v[j][0].x + acc[j][s][0].x, v[j][0].y + acc[j][s][0].y};
-          t[1] = float2{v[j][1].x + acc[j][s][1].x, v[j][1].y + acc[j][s][1].y};
+          float2 *t = tile + corner + (r0 + j * NW) * plane + off[sl];
+          t[0] = float2{v[j][0].x + acc[st][j][sl][0].x, v[j][0].y + acc[st][j][sl][0].y};
+          t[1] = float2{v[j][1].x + acc[st][j][sl][1].x, v[j][1].y + acc[st][j][sl][1].y};
         }
       }
 #pragma unroll
-      for (int j = 0; j < RW; j++) acc[j][s][0] = acc[j][s][1] = float2{0.f, 0.f};
+      for (int j = 0; j < RW; j++) acc[st][j][sl][0] = acc[st][j][sl][1] = float2{0.f, 0.f};
     }
   };
 
-  // chunk of 64 nodes, one per lane: packed corner, frac - 1/2, value.
-  // Loads run two chunks ahead for the node index (perm) and one chunk ahead
-  // for everything else, and are converted only when the chunk becomes
-  // current, so no wait for memory falls inside a chunk's work.
-  auto pos = [&](int first) { return (long)sp.y + (first + lane < sp.z ? first + lane : 0); };
-  int ckey;
-  float cfr[3];
-  float2 cf;
-  int nc[3];
-  float nfr[3];
-  float2 nf;
-  int nperm, nnperm = 0;
-  auto load_raw = [&](int first, int pidx) {
-    const long p = pos(first);
+  // chunk of 64 nodes per stream, one per lane: packed corner, frac - 1/2,
+  // value.  Loads run two chunks ahead for the node index (perm) and one
+  // chunk ahead for everything else, and are converted only when the chunk
+  // becomes current, so no wait for memory falls inside a chunk's work.
+  auto pos = [&](int st, int first) {
+    // a valid node of the work item for every lane (empty streams and lanes
+    // past the stream's end read the stream's or the item's first node)
+    const int i = first + lane;
+    return (long)sp.y + (sn[st] > 0 ? sb[st] + (i < sn[st] ? i : 0) : 0);
+  };
+  int ckey[S];
+  float cfr[S][3];
+  float2 cf[S];
+  int nc[S][3];
+  float nfr[S][3];
+  float2 nf[S];
+  int nperm[S], nnperm[S];
+  auto load_raw = [&](int st, int first, int pidx) {
+    const long p = pos(st, first);
 #pragma unroll
     for (int t = 0; t < 3; t++) {
-      nc[t] = cell[t * gg.M + p];
-      nfr[t] = frac[t * gg.M + p];
+      nc[st][t] = cell[t * gg.M + p];
+      nfr[st][t] = frac[t * gg.M + p];
     }
-    nf = fin[pidx];
+    nf[st] = fin[pidx];
   };
-  auto convert = [&]() {
+  auto convert = [&](int st) {
     int a[3];
 #pragma unroll
     for (int t = 0; t < 3; t++) {
-      a[t] = nc[t] - lo[t];
-      cfr[t] = nfr[t] - 0.5f;
+      a[t] = nc[st][t] - lo[t];
+      cfr[st][t] = nfr[st][t] - 0.5f;
     }
-    ckey = ((a[0] * E1 + a[1]) * E2 + a[2]) | (a[0] << 20);
-    cf = nf;
+    ckey[st] = ((a[0] * E1 + a[1]) * E2 + a[2]) | (a[0] << 20);
+    cf[st] = nf[st];
   };
-  load_raw(0, perm[pos(0)]);
-  nperm = perm[pos(64)];
-  convert();
-  for (int first = 0; first < sp.z; first += 64) {
-    if (first + 64 < sp.z) {
-      load_raw(first + 64, nperm);
-      nnperm = perm[pos(first + 128)];
-    }
-    const int cnt = min(64, sp.z - first);
-    for (int base = 0; base < cnt; base += G) {
-      // window values of nodes base .. base+G-1 (lanes past cnt: unused)
-      float mine[NPASS], xr[NPASS], xi[NPASS];
+  int longest = 0;
 #pragma unroll
-      for (int p = 0; p < NPASS; p++) {
-        const int src = 4 * (base + NP * p + h);
-        const float f0 = bperm_f(src, cfr[0]);
-        const float f1 = bperm_f(src, cfr[1]);
-        const float f2 = bperm_f(src, cfr[2]);
-        const float fx = bperm_f(src, cf.x);
-        const float fy = bperm_f(src, cf.y);
-        const float tt = wt == 0 ? f0 : (wt == 1 ? f1 : f2);
-        float v = coef[kPolyDeg];
+  for (int st = 0; st < S; st++) {
+    load_raw(st, 0, perm[pos(st, 0)]);
+    nperm[st] = perm[pos(st, 64)];
+    nnperm[st] = 0;
+    convert(st);
+    longest = max(longest, sn[st]);
+  }
+  for (int first = 0; first < longest; first += 64) {
+    int cnt[S];
+    int cmax = 0;
 #pragma unroll
-        for (int k = kPolyDeg - 1; k >= 0; k--) v = fmaf(v, tt, coef[k]);
-        mine[p] = v;
-        xr[p] = v * fx;
-        xi[p] = v * fy;
+    for (int st = 0; st < S; st++) {
+      if (first + 64 < sn[st]) {
+        load_raw(st, first + 64, nperm[st]);
+        nnperm[st] = perm[pos(st, first + 128)];
       }
-      float w0[G][SLOTS], w1[G][SLOTS];
+      cnt[st] = max(0, min(64, sn[st] - first));
+      cmax = max(cmax, cnt[st]);
+    }
+    for (int base = 0; base < cmax; base += G) {
+      // window values of nodes base .. base+G-1 of every stream (lanes past
+      // the stream's count: unused)
+      float xr[S][NPASS], xi[S][NPASS];
+      float w0[S][G][SLOTS], w1[S][G][SLOTS];
 #pragma unroll
-      for (int k = 0; k < G; k++) {
-        const int p = k / NP, hk = k % NP;
+      for (int st = 0; st < S; st++) {
+        float mine[NPASS];
 #pragma unroll
-        for (int s = 0; s < SLOTS; s++) {
-          const float u = bperm_f(adr_u[hk][s], mine[p]);
-          w0[k][s] = u * bperm_f(adr_v0[hk][s], mine[p]);
-          w1[k][s] = u * bperm_f(adr_v1[hk][s], mine[p]);
+        for (int p = 0; p < NPASS; p++) {
+          const int src = 4 * (base + NP * p + h);
+          const float f0 = bperm_f(src, cfr[st][0]);
+          const float f1 = bperm_f(src, cfr[st][1]);
+          const float f2 = bperm_f(src, cfr[st][2]);
+          const float fx = bperm_f(src, cf[st].x);
+          const float fy = bperm_f(src, cf[st].y);
+          const float tt = wt == 0 ? f0 : (wt == 1 ? f1 : f2);
+          float v = coef[kPolyDeg];
+#pragma unroll
+          for (int k = kPolyDeg - 1; k >= 0; k--) v = fmaf(v, tt, coef[k]);
+          mine[p] = v;
+          xr[st][p] = v * fx;
+          xi[st][p] = v * fy;
+        }
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+          const int p = k / NP, hk = k % NP;
+#pragma unroll
+          for (int sl = 0; sl < SLOTS; sl++) {
+            const float u = bperm_f(adr_u[hk][sl], mine[p]);
+            w0[st][k][sl] = u * bperm_f(adr_v0[hk][sl], mine[p]);
+            w1[st][k][sl] = u * bperm_f(adr_v1[hk][sl], mine[p]);
+          }
         }
       }
-      const int ng = min(G, cnt - base);
 #pragma unroll
       for (int k = 0; k < G; k++) {
-        if (k < ng) {
-          const int p = k / NP, hk = k % NP;
-          const int key = __builtin_amdgcn_readlane(ckey, base + k);
-          if (key != cur) {
-            if (cur >= 0) flush();
-            cur = key;
-          }
-          const int r0 = ((wv - (key >> 20)) % NW + NW) % NW;
 #pragma unroll
-          for (int j = 0; j < RW; j++) {
-            const int src = hk * 32 + r0 + j * NW;
-            const float sr = rlane_f(xr[p], src);
-            const float si = rlane_f(xi[p], src);
+        for (int st = 0; st < S; st++) {
+          if (base + k < cnt[st]) {
+            const int p = k / NP, hk = k % NP;
+            const int key = __builtin_amdgcn_readlane(ckey[st], base + k);
+            if (key != cur[st]) {
+              if (cur[st] >= 0) flush(st);
+              cur[st] = key;
+            }
+            const int r0 = ((wv - (key >> 20)) % NW + NW) % NW;
 #pragma unroll
-            for (int s = 0; s < SLOTS; s++) {
-              acc[j][s][0].x = fmaf(w0[k][s], sr, acc[j][s][0].x);
-              acc[j][s][0].y = fmaf(w0[k][s], si, acc[j][s][0].y);
-              acc[j][s][1].x = fmaf(w1[k][s], sr, acc[j][s][1].x);
-              acc[j][s][1].y = fmaf(w1[k][s], si, acc[j][s][1].y);
+            for (int j = 0; j < RW; j++) {
+              const int src = hk * 32 + r0 + j * NW;
+              const float sr = rlane_f(xr[st][p], src);
+              const float si = rlane_f(xi[st][p], src);
+#pragma unroll
+              for (int sl = 0; sl < SLOTS; sl++) {
+                acc[st][j][sl][0].x = fmaf(w0[st][k][sl], sr, acc[st][j][sl][0].x);
+                acc[st][j][sl][0].y = fmaf(w0[st][k][sl], si, acc[st][j][sl][0].y);
+                acc[st][j][sl][1].x = fmaf(w1[st][k][sl], sr, acc[st][j][sl][1].x);
+                acc[st][j][sl][1].y = fmaf(w1[st][k][sl], si, acc[st][j][sl][1].y);
+              }
             }
           }
         }
       }
     }
-    if (first + 64 < sp.z) {
-      convert();
-      nperm = nnperm;
+#pragma unroll
+    for (int st = 0; st < S; st++) {
+      if (first + 64 < sn[st]) {
+        convert(st);
+        nperm[st] = nnperm[st];
+      }
     }
   }
-  if (cur >= 0) flush();
+#pragma unroll
+  for (int st = 0; st < S; st++)
+    if (cur[st] >= 0) flush(st);
   __syncthreads();
   flush_tile<float, 3>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, threadIdx.x,
                        blockDim.x);
