@@ -233,7 +233,8 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   auto variant = [](const char *name, int dflt) {
     const char *v = std::getenv(name);
     if (!v) return dflt;
-    return std::strcmp(v, "runs2s") == 0 ? 6
+    return std::strcmp(v, "runs2st") == 0  ? 7
+           : std::strcmp(v, "runs2s") == 0 ? 6
            : std::strcmp(v, "runs5") == 0 ? 5
            : std::strcmp(v, "runs") == 0 ? 4
            : std::strcmp(v, "rows") == 0 ? 3
@@ -246,7 +247,7 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   // float: plain FP32 read-modify-write by wavefronts that own disjoint
   // cells (row-partitioned with same-cell runs summed in registers in 3-D,
   // private tiles in 2-D); double: wavefronts sharing a tile with ds_add_f64
-  spread_variant = variant("NFFT_HIP_SPREAD", sizeof(R) == 4 ? (d == 3 ? 4 : 2) : 1);
+  spread_variant = variant("NFFT_HIP_SPREAD", sizeof(R) == 4 ? (d == 3 ? 6 : 2) : 1);
 
   // FFT: twiddles per dimension, then the pruned passes of both directions
   const int cap = sizeof(R) == 4 ? 4096 : 2048;  // elements per workgroup
@@ -452,7 +453,7 @@ void Engine<R>::autotune() {
     // float candidates accumulate in plain FP32 (the fixed-point LDS
     // variants are left out); double ones use ds_add_f64
     std::vector<int> cand;
-    if (sizeof(R) == 4 && d == 3) cand = {4, 6, 5, 3};
+    if (sizeof(R) == 4 && d == 3) cand = {7, 6, 4, 5, 3};
     else if (sizeof(R) == 8 && d >= 2) cand = {1, 0};
     float best = 1e30f;
     int pick = spread_variant;

@@ -1287,6 +1287,7 @@ template <typename R> struct BKernelTable {
   fn spread_runs[kMaxStaticM + 1];            // d = 3, float only, 128 threads
   fn spread_runs_wide[kMaxStaticM + 1];       // same, runs_wide_nw(W) wavefronts
   fn spread_runs_2s[kMaxStaticM + 1];         // same, two node streams per wavefront
+  fn spread_runs_2st[kMaxStaticM + 1];        // same, row weights through LDS
   fn interp_rt[kMaxDim][2];  // [d-1][tile]
   fn spread_rt[kMaxDim][2];
 };
@@ -1302,9 +1303,10 @@ void bk_fill_one(BKernelTable<R> &k) {
   }
   if constexpr (D == 3 && D * (2 * M_ + 2) <= 64) k.spread_rows[M_] = k_spread_rows2<R, 2 * M_ + 2, 2>;
   if constexpr (std::is_same_v<R, float> && D == 3 && D * (2 * M_ + 2) <= 64) {
-    k.spread_runs[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 1>;
-    k.spread_runs_wide[M_] = k_spread_runs<2 * M_ + 2, runs_wide_nw(2 * M_ + 2), 8, 1>;
-    k.spread_runs_2s[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2>;
+    k.spread_runs[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 1, false>;
+    k.spread_runs_wide[M_] = k_spread_runs<2 * M_ + 2, runs_wide_nw(2 * M_ + 2), 8, 1, false>;
+    k.spread_runs_2s[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false>;
+    k.spread_runs_2st[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, true>;
   }
 }
 
@@ -1348,6 +1350,12 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
   threads = 256;
   if (tile && gg.m >= 1 && gg.m <= kMaxStaticM) {
     const int t = gg.d - 1;
+    if (spread && variant == 7 && gg.d == 3 && k.spread_runs_2st[gg.m]) {
+      threads = 128;
+      // row-weight table: [2 wavefronts][2 streams][8 nodes][W/2 rows]
+      lds = bytes + sizeof(cplx<R>) * 2 * 2 * 8 * (gg.m + 1);
+      return k.spread_runs_2st[gg.m];
+    }
     if (spread && variant == 6 && gg.d == 3 && k.spread_runs_2s[gg.m]) {
       threads = 128;
       return k.spread_runs_2s[gg.m];

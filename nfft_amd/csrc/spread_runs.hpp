@@ -46,7 +46,10 @@ constexpr int runs_wide_nw(int W) {
 // gives the wavefront S independent dependency chains.  Streams of one
 // wavefront may touch the same tile cells; a wavefront's LDS operations
 // complete in program order, so their read-modify-writes do not race.
-template <int W, int NW, int G, int S>
+// TAB: the row weights psi0 * f_j of this wavefront's rows go through a small
+// LDS table (written by the lanes that evaluated them, read back as uniform
+// loads) instead of one readlane per row and component.
+template <int W, int NW, int G, int S, bool TAB>
 __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restrict__ fin,
                                                          float2 *__restrict__ g,
                                                          const int *__restrict__ perm,
@@ -116,6 +119,8 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
       sn[st] = min(sp.z, (st + 1) * part) - sb[st];
     }
   }
+  // row-weight table after the tile: [wavefront][stream][node of group][row]
+  float2 *rtab = tile + tot + (long)wv * S * G * RW;
   // run accumulators per stream: [row j][slot][pair cell], complex
   float2 acc[S][RW][SLOTS][2];
 #pragma unroll
@@ -235,6 +240,14 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
           mine[p] = v;
           xr[st][p] = v * fx;
           xi[st][p] = v * fy;
+          if constexpr (TAB) {
+            // lanes holding a row tap of this wavefront's rows store its weight
+            const int a0 = __builtin_amdgcn_ds_bpermute(src, ckey[st]) >> 20;
+            const int r0n = ((wv - a0) % NW + NW) % NW;
+            const int d = ww - r0n;
+            if (wt == 0 && li < W && d >= 0 && d % NW == 0)
+              rtab[(st * G + NP * p + h) * RW + d / NW] = float2{xr[st][p], xi[st][p]};
+          }
         }
 #pragma unroll
         for (int k = 0; k < G; k++) {
@@ -259,11 +272,16 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
               cur[st] = key;
             }
             const int r0 = ((wv - (key >> 20)) % NW + NW) % NW;
+            float2 rw[RW];
+            if constexpr (TAB) {
+#pragma unroll
+              for (int j = 0; j < RW; j++) rw[j] = rtab[(st * G + k) * RW + j];
+            }
 #pragma unroll
             for (int j = 0; j < RW; j++) {
               const int src = hk * 32 + r0 + j * NW;
-              const float sr = rlane_f(xr[st][p], src);
-              const float si = rlane_f(xi[st][p], src);
+              const float sr = TAB ? rw[j].x : rlane_f(xr[st][p], src);
+              const float si = TAB ? rw[j].y : rlane_f(xi[st][p], src);
 #pragma unroll
               for (int sl = 0; sl < SLOTS; sl++) {
                 acc[st][j][sl][0].x = fmaf(w0[st][k][sl], sr, acc[st][j][sl][0].x);
