@@ -233,7 +233,9 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   auto variant = [](const char *name, int dflt) {
     const char *v = std::getenv(name);
     if (!v) return dflt;
-    return std::strcmp(v, "runs2st") == 0  ? 7
+    return std::strcmp(v, "runs1w3s") == 0  ? 9
+           : std::strcmp(v, "runs1w2s") == 0 ? 8
+           : std::strcmp(v, "runs2st") == 0 ? 7
            : std::strcmp(v, "runs2s") == 0 ? 6
            : std::strcmp(v, "runs5") == 0 ? 5
            : std::strcmp(v, "runs") == 0 ? 4
@@ -453,7 +455,7 @@ void Engine<R>::autotune() {
     // float candidates accumulate in plain FP32 (the fixed-point LDS
     // variants are left out); double ones use ds_add_f64
     std::vector<int> cand;
-    if (sizeof(R) == 4 && d == 3) cand = {7, 6, 4, 5, 3};
+    if (sizeof(R) == 4 && d == 3) cand = {6, 8, 9, 7, 4, 5, 3};
     else if (sizeof(R) == 8 && d >= 2) cand = {1, 0};
     float best = 1e30f;
     int pick = spread_variant;

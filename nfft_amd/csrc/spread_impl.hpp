@@ -1288,6 +1288,8 @@ template <typename R> struct BKernelTable {
   fn spread_runs_wide[kMaxStaticM + 1];       // same, runs_wide_nw(W) wavefronts
   fn spread_runs_2s[kMaxStaticM + 1];         // same, two node streams per wavefront
   fn spread_runs_2st[kMaxStaticM + 1];        // same, row weights through LDS
+  fn spread_runs_1w2s[kMaxStaticM + 1];       // one wavefront per tile, two streams
+  fn spread_runs_1w3s[kMaxStaticM + 1];       // one wavefront per tile, three streams
   fn interp_rt[kMaxDim][2];  // [d-1][tile]
   fn spread_rt[kMaxDim][2];
 };
@@ -1307,6 +1309,8 @@ void bk_fill_one(BKernelTable<R> &k) {
     k.spread_runs_wide[M_] = k_spread_runs<2 * M_ + 2, runs_wide_nw(2 * M_ + 2), 8, 1, false>;
     k.spread_runs_2s[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false>;
     k.spread_runs_2st[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, true>;
+    k.spread_runs_1w2s[M_] = k_spread_runs<2 * M_ + 2, 1, 8, 2, false>;
+    k.spread_runs_1w3s[M_] = k_spread_runs<2 * M_ + 2, 1, 8, 3, false>;
   }
 }
 
@@ -1350,6 +1354,10 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
   threads = 256;
   if (tile && gg.m >= 1 && gg.m <= kMaxStaticM) {
     const int t = gg.d - 1;
+    if (spread && (variant == 8 || variant == 9) && gg.d == 3 && k.spread_runs_1w2s[gg.m]) {
+      threads = 64;
+      return variant == 8 ? k.spread_runs_1w2s[gg.m] : k.spread_runs_1w3s[gg.m];
+    }
     if (spread && variant == 7 && gg.d == 3 && k.spread_runs_2st[gg.m]) {
       threads = 128;
       // row-weight table: [2 wavefronts][2 streams][8 nodes][W/2 rows]
