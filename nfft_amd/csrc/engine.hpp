@@ -19,7 +19,7 @@
 
 namespace nfft_hip {
 
-constexpr int kMaxDim = 3;
+constexpr int kMaxDim = 4;
 
 // Geometry shared by the node-binning and B kernels.
 struct BinGeom {
@@ -78,14 +78,14 @@ template <typename R> struct FftPass {
   int Nband;      // N_t
   int in_mode, out_mode;
   int nod;        // other dimensions (d - 1), outer first
-  int cnt[2];     // lines along each other dimension
-  int band[2];    // 1: that coordinate runs over the band
-  int N_od[2], n_od[2];
-  long gstr[2];   // grid strides of the other dimensions
-  long fstr[2];   // f_hat strides of the other dimensions
+  int cnt[kMaxDim - 1];     // lines along each other dimension
+  int band[kMaxDim - 1];    // 1: that coordinate runs over the band
+  int N_od[kMaxDim - 1], n_od[kMaxDim - 1];
+  long gstr[kMaxDim - 1];   // grid strides of the other dimensions
+  long fstr[kMaxDim - 1];   // f_hat strides of the other dimensions
   long fstr_ax;   // f_hat stride of dimension t
   const R *phi_ax;
-  const R *phi_od[2];
+  const R *phi_od[kMaxDim - 1];
   long nlines;
   int lines_per_block;
   int nstages;
@@ -97,8 +97,8 @@ template <typename R> struct FftPass {
 template <typename R> struct Engine {
   // geometry
   int d = 0;
-  long N[kMaxDim] = {1, 1, 1};
-  long n[kMaxDim] = {1, 1, 1};
+  long N[kMaxDim] = {1, 1, 1, 1};
+  long n[kMaxDim] = {1, 1, 1, 1};
   long N_total = 0, n_total = 0, M = 0;
   int m = 0;
   bool direct_only = false;  // some N_t <= m or n_t <= 2m+2: NDFT path

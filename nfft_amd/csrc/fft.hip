@@ -234,7 +234,7 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
   // other dimensions
   for (int ll = threadIdx.x; ll < TL; ll += blockDim.x) {
     long L = L0 + ll;
-    int c[2] = {0, 0};
+    int c[kMaxDim - 1] = {};
     for (int s = P.nod - 1; s >= 0; s--) {
       c[s] = (int)(L % P.cnt[s]);
       L /= P.cnt[s];

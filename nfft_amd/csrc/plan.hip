@@ -182,7 +182,7 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   // GPU (1-D 256 cells, 2-D n/32 clamped to [16, 64], 3-D 16), then shrink
   // the largest edge until the LDS tile fits the budget
   const size_t budget = 48 * 1024;
-  int bs[kMaxDim] = {1, 1, 1};
+  int bs[kMaxDim] = {1, 1, 1, 1};
   for (int t = 0; t < d; t++) {
     const long init = d == 1 ? 256 : d == 2 ? std::min<long>(64, std::max<long>(16, n[t] / 32)) : 16;
     bs[t] = (int)std::min<long>(n[t], init);
@@ -219,7 +219,8 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
       tb *= (size_t)geom.E[t];
       fits = fits && geom.E[t] <= n[t];
     }
-    tile_scratch = fits && tb <= 64 * 1024;
+    // (the tile reductions cover d <= 3; d = 4 adds its tiles with atomics)
+    tile_scratch = fits && tb <= 64 * 1024 && d <= 3;
     bool zoned = tile_scratch;
     long core = 1;
     for (int t = 0; t < d; t++) {
@@ -624,7 +625,7 @@ struct PlanApi {
       family<R>::fatal("nfft: no HIP device available (this library runs on the GPU only)");
     if (p->flags & NFFT_OMP_BLOCKWISE_ADJOINT) p->flags |= NFFT_SORT_NODES;
     if (p->d < 1 || p->d > kMaxDim)
-      family<R>::fatal("nfft: dimension d = %td is not supported (1 <= d <= 3)", p->d);
+      family<R>::fatal("nfft: dimension d = %td is not supported (1 <= d <= 4)", p->d);
     p->N_total = 1;
     p->n_total = 1;
     for (NFFT_INT t = 0; t < p->d; t++) {

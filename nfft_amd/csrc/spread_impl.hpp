@@ -93,7 +93,12 @@ __device__ __forceinline__ void flush_tile(Get get, cplx<R> *__restrict__ tiles,
     for (int t = 0; t < D; t++) zcount *= 3;
     long o = 0;
     for (int zz = 0; zz < zcount; zz++) {
-      int st[3] = {0, 0, 0}, sz[3] = {1, 1, 1};
+      int st[kMaxDim], sz[kMaxDim];
+#pragma unroll
+      for (int t = 0; t < kMaxDim; t++) {
+        st[t] = 0;
+        sz[t] = 1;
+      }
       int r = zz;
 #pragma unroll
       for (int t = D - 1; t >= 0; t--) {
@@ -906,7 +911,7 @@ __global__ __launch_bounds__(256) void k_interp_rt(const cplx<R> *__restrict__ g
         ar += psi[0][i0] * sr;
         ai += psi[0][i0] * si;
       }
-    } else {
+    } else if constexpr (D == 3) {
       for (int i0 = 0; i0 < W; i0++) {
         R qr = 0, qi = 0;
         for (int i1 = 0; i1 < W; i1++) {
@@ -922,6 +927,34 @@ __global__ __launch_bounds__(256) void k_interp_rt(const cplx<R> *__restrict__ g
         }
         ar += psi[0][i0] * qr;
         ai += psi[0][i0] * qi;
+      }
+    } else {
+      // any d: odometer over the taps of the first d-1 dimensions, last
+      // dimension innermost (the generic path of the reference,
+      // kernel/nfft/nfft.c:630-1100)
+      int lt[D];
+      for (int t = 0; t < D; t++) lt[t] = 0;
+      long nrows = 1;
+      for (int t = 0; t < D - 1; t++) nrows *= W;
+      for (long q = 0; q < nrows; q++) {
+        R wgt = R(1);
+        long row = 0;
+        for (int t = 0; t < D - 1; t++) {
+          wgt *= psi[t][lt[t]];
+          row += idx[t][lt[t]] * stride[t];
+        }
+        R sr = 0, si = 0;
+        for (int w = 0; w < W; w++) {
+          const cplx<R> v = src[row + idx[D - 1][w]];
+          sr += psi[D - 1][w] * v.x;
+          si += psi[D - 1][w] * v.y;
+        }
+        ar += wgt * sr;
+        ai += wgt * si;
+        for (int t = D - 2; t >= 0; t--) {
+          if (++lt[t] < W) break;
+          lt[t] = 0;
+        }
       }
     }
     fout[perm[p]] = cplx<R>{ar, ai};
@@ -983,13 +1016,33 @@ __global__ __launch_bounds__(256) void k_spread_rt(const cplx<R> *__restrict__ f
         const R wr = psi[0][i0] * fj.x, wi = psi[0][i0] * fj.y;
         for (int w = 0; w < W; w++) put(row + idx[1][w], psi[1][w] * wr, psi[1][w] * wi);
       }
-    } else {
+    } else if constexpr (D == 3) {
       for (int i0 = 0; i0 < W; i0++) {
         for (int i1 = 0; i1 < W; i1++) {
           const long row = idx[0][i0] * stride[0] + idx[1][i1] * stride[1];
           const R w01 = psi[0][i0] * psi[1][i1];
           const R wr = w01 * fj.x, wi = w01 * fj.y;
           for (int w = 0; w < W; w++) put(row + idx[2][w], psi[2][w] * wr, psi[2][w] * wi);
+        }
+      }
+    } else {
+      // any d: odometer over the first d-1 dimensions, last innermost
+      int lt[D];
+      for (int t = 0; t < D; t++) lt[t] = 0;
+      long nrows = 1;
+      for (int t = 0; t < D - 1; t++) nrows *= W;
+      for (long q = 0; q < nrows; q++) {
+        R wgt = R(1);
+        long row = 0;
+        for (int t = 0; t < D - 1; t++) {
+          wgt *= psi[t][lt[t]];
+          row += idx[t][lt[t]] * stride[t];
+        }
+        const R wr = wgt * fj.x, wi = wgt * fj.y;
+        for (int w = 0; w < W; w++) put(row + idx[D - 1][w], psi[D - 1][w] * wr, psi[D - 1][w] * wi);
+        for (int t = D - 2; t >= 0; t--) {
+          if (++lt[t] < W) break;
+          lt[t] = 0;
         }
       }
     }
@@ -1330,6 +1383,11 @@ BKernelTable<R> make_bkernel_table() {
   bk_fill_dim<R, 1>(t, seq{});
   bk_fill_dim<R, 2>(t, seq{});
   bk_fill_dim<R, 3>(t, seq{});
+  // d = 4: the generic runtime-m kernels only
+  t.interp_rt[3][0] = k_interp_rt<R, 4, false>;
+  t.interp_rt[3][1] = k_interp_rt<R, 4, true>;
+  t.spread_rt[3][0] = k_spread_rt<R, 4, false>;
+  t.spread_rt[3][1] = k_spread_rt<R, 4, true>;
   return t;
 }
 
@@ -1352,7 +1410,7 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
   const bool tile = bytes <= kTileBudget;
   lds = tile ? bytes : 0;
   threads = 256;
-  if (tile && gg.m >= 1 && gg.m <= kMaxStaticM) {
+  if (tile && gg.d <= 3 && gg.m >= 1 && gg.m <= kMaxStaticM) {
     const int t = gg.d - 1;
     if (spread && (variant == 8 || variant == 9) && gg.d == 3 && k.spread_runs_1w2s[gg.m]) {
       threads = 64;

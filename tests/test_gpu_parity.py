@@ -259,3 +259,20 @@ def test_measured_plan_vs_oracle(case, precision):
     tol = 1e-12 if precision == "double" else 1e-5
     assert rel_errs(fg, O.nfft_trafo(N, nn, m, xr, fhr))[0] < tol
     assert rel_errs(hg, O.nfft_adjoint(N, nn, m, xr, fr))[0] < tol
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("m,N,n", [(2, [6, 4, 6, 4], [16, 8, 16, 8]), (3, [6, 6, 4, 6], [12, 12, 10, 12])])
+def test_four_dimensional(precision, m, N, n):
+    """d = 4 goes through the generic-d kernels (tap odometer, reference
+    kernel/nfft/nfft.c:630-1100) and four pruned FFT passes."""
+    x = W.uniform(400, 4, seed=81)
+    fh = W.complex_normal(int(np.prod(N)), seed=82)
+    f = W.complex_normal(400, seed=83)
+    (fg, fr, fhr), (hg, hr, frr) = run_pair(N, x, fh, f, m, precision, n=n)
+    l2t, _ = rel_errs(fg, fr)
+    l2a, _ = rel_errs(hg, hr)
+    print("4-D m=%d %s trafo l2 %.2e adjoint l2 %.2e" % (m, precision, l2t, l2a))
+    tol = 1e-12 if precision == "double" else 1e-5
+    assert l2t < tol and l2a < tol
