@@ -45,7 +45,11 @@ template <typename R> struct WinParams {
   // degree kPolyDeg polynomial in (frac - 1/2), coefficients lowest first at
   // poly[(t * (2m+2) + w) * kPolyCoef + k]; fitted on the host in long double
   // (Chebyshev interpolation, |error| < 1e-9 of the peak for m <= 8).
+  // The table is mirror-symmetric: tap 2m+1-w at t is tap w at -t, so its
+  // coefficients are those of tap w with odd powers negated.
   const R *poly;
+  // every dimension has the same sigma_t, hence the same tap polynomials
+  int poly_shared;
 };
 constexpr int kPolyDeg = 10;
 constexpr int kPolyCoef = kPolyDeg + 1;
@@ -142,10 +146,16 @@ template <typename R> struct Engine {
   BinGeom geom{};
   WinParams<R> win{};
   int max_per_sub = 1024;
-  // B kernel shapes: 0 lane per node, 1 wave per node (d = 2, 3).  Set at
-  // plan creation; NFFT_HIP_INTERP / NFFT_HIP_SPREAD = lane | wave override.
+  // B kernel shapes: 0 lane per node (lane), 1 wave per node (wave); for
+  // spreading also 2 private tile per wavefront (priv), 3 row-partitioned
+  // tile (rows), and the FP32 3-D run-merging kernels 4 (runs), 5 (runs5),
+  // 6 (runs2s), 7 (runs2st), 8 (runs1w2s), 9 (runs1w3s), 10 (runs2sbc).
+  // Set at plan creation; NFFT_HIP_INTERP / NFFT_HIP_SPREAD = name override.
   int interp_variant = 0;
   int spread_variant = 1;
+  // lane-per-node FP32 interpolation evaluates tap pairs from the shared
+  // mirror-symmetric window table when win.poly_shared (NFFT_HIP_SYMWIN=0: off)
+  int sym_window = 1;
   std::vector<FftAxis<R>> axes;
   std::vector<FftPass<R>> fwd_passes;  // trafo: D fused into the first pass
   std::vector<FftPass<R>> bwd_passes;  // adjoint: D^T fused into the last pass

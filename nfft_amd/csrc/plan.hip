@@ -12,6 +12,7 @@
 // plan's current pointers (the solver swaps them between calls).  Node data
 // are taken from x at precompute time when a PRE_* psi flag is set, and from
 // the current x on every call otherwise, as the reference does.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -67,6 +68,7 @@ static long double psi_scaled(long double y, int m, long double b) {
 template <typename R>
 static void fit_window_polys(int m, long double b, std::vector<R> &out) {
   const int K = 32, DEG = kPolyDeg;
+  const size_t first = out.size();
   for (int w = 0; w < 2 * m + 2; w++) {
     long double f[K], u[K];
     for (int j = 0; j < K; j++) {
@@ -102,6 +104,12 @@ static void fit_window_polys(int m, long double b, std::vector<R> &out) {
     // u = 2t: coefficient of t^i is mono[i] 2^i
     for (int i = 0; i <= DEG; i++) out.push_back((R)(mono[i] * ldexpl(1.0L, i)));
   }
+  // psi is even, so psi(frac + m - (2m+1-w)) = psi((1 - frac) + m - w): the
+  // upper taps are the lower ones at -t.  Written as exact mirrors so that
+  // kernels may evaluate a tap pair from one polynomial's even and odd parts.
+  for (int w = m + 1; w < 2 * m + 2; w++)
+    for (int i = 0; i <= DEG; i++)
+      out[first + w * (DEG + 1) + i] = (i % 2 ? -1 : 1) * out[first + (2 * m + 1 - w) * (DEG + 1) + i];
 }
 
 // factor n into FFT radices, largest first; returns false if a prime > 13 remains
@@ -176,6 +184,10 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     NFFT_HIP_CHECK(R, hipMalloc(&poly, sizeof(R) * pc.size()));
     NFFT_HIP_CHECK(R, hipMemcpy(poly, pc.data(), sizeof(R) * pc.size(), hipMemcpyHostToDevice));
     win.poly = poly;
+    const size_t per_dim = pc.size() / d;
+    win.poly_shared = 1;
+    for (int t = 1; t < d; t++)
+      if (!std::equal(pc.begin(), pc.begin() + per_dim, pc.begin() + t * per_dim)) win.poly_shared = 0;
   }
 
   // bins: start from an edge that gives the grid enough bins to fill the
@@ -235,6 +247,7 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     const char *v = std::getenv(name);
     if (!v) return dflt;
     return std::strcmp(v, "runs1w3s") == 0  ? 9
+           : std::strcmp(v, "runs2sbc") == 0 ? 10
            : std::strcmp(v, "runs1w2s") == 0 ? 8
            : std::strcmp(v, "runs2st") == 0 ? 7
            : std::strcmp(v, "runs2s") == 0 ? 6
@@ -247,10 +260,11 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
                                          : dflt;
   };
   interp_variant = variant("NFFT_HIP_INTERP", 0);
+  if (const char *v = std::getenv("NFFT_HIP_SYMWIN")) sym_window = std::atoi(v) != 0;
   // float: plain FP32 read-modify-write by wavefronts that own disjoint
   // cells (row-partitioned with same-cell runs summed in registers in 3-D,
   // private tiles in 2-D); double: wavefronts sharing a tile with ds_add_f64
-  spread_variant = variant("NFFT_HIP_SPREAD", sizeof(R) == 4 ? (d == 3 ? 6 : 2) : 1);
+  spread_variant = variant("NFFT_HIP_SPREAD", sizeof(R) == 4 ? (d == 3 ? 10 : 2) : 1);
 
   // FFT: twiddles per dimension, then the pruned passes of both directions
   const int cap = sizeof(R) == 4 ? 4096 : 2048;  // elements per workgroup
@@ -456,7 +470,7 @@ void Engine<R>::autotune() {
     // float candidates accumulate in plain FP32 (the fixed-point LDS
     // variants are left out); double ones use ds_add_f64
     std::vector<int> cand;
-    if (sizeof(R) == 4 && d == 3) cand = {6, 8, 9, 7, 4, 5, 3};
+    if (sizeof(R) == 4 && d == 3) cand = {10, 6, 8, 9, 7, 4, 5, 3};
     else if (sizeof(R) == 8 && d >= 2) cand = {1, 0};
     float best = 1e30f;
     int pick = spread_variant;

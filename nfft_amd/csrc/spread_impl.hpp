@@ -228,6 +228,39 @@ __device__ __forceinline__ void node_window(const int *__restrict__ cell,
   }
 }
 
+// FP32 window of a node when every dimension shares one mirror-symmetric
+// tap table (wp.poly_shared): taps w and W-1-w come from the even part E and
+// odd part O of tap w's polynomial, psi = E(t^2) +- t O(t^2).  The W/2
+// polynomials of dimension 0 (W/2 * kPolyCoef values) stay in scalar
+// registers for the whole kernel instead of the d * W per-tap sets, and a
+// tap pair costs kPolyDeg + 1 FMAs instead of 2 kPolyDeg.
+template <int D, int W>
+__device__ __forceinline__ void node_window_sym(const int *__restrict__ cell,
+                                                const float *__restrict__ frac, long p, long M,
+                                                const WinParams<float> &wp, int (&c)[D],
+                                                float (&psi)[D][W]) {
+  constexpr int KE = kPolyDeg - (kPolyDeg % 2);  // highest even power
+  constexpr int KO = kPolyDeg - 1 + (kPolyDeg % 2);  // highest odd power
+#pragma unroll
+  for (int t = 0; t < D; t++) {
+    c[t] = cell[t * M + p];
+    const float tt = frac[t * M + p] - 0.5f;
+    const float t2 = tt * tt;
+#pragma unroll
+    for (int w = 0; w < W / 2; w++) {
+      const float *pc = wp.poly + w * kPolyCoef;
+      float e = pc[KE];
+#pragma unroll
+      for (int k = KE - 2; k >= 0; k -= 2) e = fmaf(e, t2, pc[k]);
+      float o = pc[KO];
+#pragma unroll
+      for (int k = KO - 2; k >= 1; k -= 2) o = fmaf(o, t2, pc[k]);
+      psi[t][w] = fmaf(tt, o, e);
+      psi[t][W - 1 - w] = fmaf(-tt, o, e);
+    }
+  }
+}
+
 // Stages the grid region of a 3-D work item (E0 x E1 x E2 cells from
 // lo - m, periodic) into LDS.  Each thread keeps one position along the last
 // dimension and walks whole rows, so per element it does one table lookup
@@ -277,7 +310,7 @@ __device__ __forceinline__ void load_region_3d(const cplx<R> *__restrict__ g, cp
   }
 }
 
-template <typename R, int D, int W>
+template <typename R, int D, int W, bool SYM = false>
 __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__ g,
                                                      cplx<R> *__restrict__ fout,
                                                      const int *__restrict__ perm,
@@ -306,7 +339,8 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
     const long p = (long)sp.y + i;
     int c[D];
     R psi[D][W];
-    node_window<R, D, W>(cell, frac, p, gg.M, wp, c, psi);
+    if constexpr (SYM) node_window_sym<D, W>(cell, frac, p, gg.M, wp, c, psi);
+    else node_window<R, D, W>(cell, frac, p, gg.M, wp, c, psi);
     R ar = 0, ai = 0;
     if constexpr (D == 1) {
       const cplx<R> *row = tile + (c[0] - lo[0]);
@@ -1332,6 +1366,7 @@ template <typename R> struct BKernelTable {
   using fn = void (*)(const cplx<R> *, cplx<R> *, const int *, const int *, const R *,
                       const int4 *, BinGeom, WinParams<R>, cplx<R> *);
   fn interp_static[kMaxDim][kMaxStaticM + 1];
+  fn interp_sym[kMaxDim][kMaxStaticM + 1];   // float only, shared tap table
   fn spread_static[kMaxDim][kMaxStaticM + 1];
   fn interp_wave[kMaxDim][kMaxStaticM + 1];  // d = 2, 3 only
   fn spread_wave[kMaxDim][kMaxStaticM + 1];
@@ -1341,6 +1376,7 @@ template <typename R> struct BKernelTable {
   fn spread_runs_wide[kMaxStaticM + 1];       // same, runs_wide_nw(W) wavefronts
   fn spread_runs_2s[kMaxStaticM + 1];         // same, two node streams per wavefront
   fn spread_runs_2st[kMaxStaticM + 1];        // same, row weights through LDS
+  fn spread_runs_2sbc[kMaxStaticM + 1];       // same, row weights by DPP row broadcast
   fn spread_runs_1w2s[kMaxStaticM + 1];       // one wavefront per tile, two streams
   fn spread_runs_1w3s[kMaxStaticM + 1];       // one wavefront per tile, three streams
   fn interp_rt[kMaxDim][2];  // [d-1][tile]
@@ -1350,6 +1386,7 @@ template <typename R> struct BKernelTable {
 template <typename R, int D, int M_>
 void bk_fill_one(BKernelTable<R> &k) {
   k.interp_static[D - 1][M_] = k_interp_tile<R, D, 2 * M_ + 2>;
+  if constexpr (std::is_same_v<R, float>) k.interp_sym[D - 1][M_] = k_interp_tile<R, D, 2 * M_ + 2, true>;
   k.spread_static[D - 1][M_] = k_spread_tile<R, D, 2 * M_ + 2>;
   if constexpr (D >= 2 && D * (2 * M_ + 2) <= 64) {
     k.interp_wave[D - 1][M_] = k_bwave_tile<R, D, 2 * M_ + 2, false>;
@@ -1362,6 +1399,7 @@ void bk_fill_one(BKernelTable<R> &k) {
     k.spread_runs_wide[M_] = k_spread_runs<2 * M_ + 2, runs_wide_nw(2 * M_ + 2), 8, 1, false>;
     k.spread_runs_2s[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false>;
     k.spread_runs_2st[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, true>;
+    k.spread_runs_2sbc[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false, true>;
     k.spread_runs_1w2s[M_] = k_spread_runs<2 * M_ + 2, 1, 8, 2, false>;
     k.spread_runs_1w3s[M_] = k_spread_runs<2 * M_ + 2, 1, 8, 3, false>;
   }
@@ -1405,7 +1443,8 @@ constexpr size_t kTileBudget = 64 * 1024;
 // tile (spreading only); falls back to what exists for (d, m)
 template <typename R>
 typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeom &gg,
-                                          bool spread, int variant, size_t &lds, int &threads) {
+                                          bool spread, int variant, size_t &lds, int &threads,
+                                          bool shared_poly = false) {
   const size_t bytes = tile_bytes<R>(gg);
   const bool tile = bytes <= kTileBudget;
   lds = tile ? bytes : 0;
@@ -1415,6 +1454,10 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
     if (spread && (variant == 8 || variant == 9) && gg.d == 3 && k.spread_runs_1w2s[gg.m]) {
       threads = 64;
       return variant == 8 ? k.spread_runs_1w2s[gg.m] : k.spread_runs_1w3s[gg.m];
+    }
+    if (spread && variant == 10 && gg.d == 3 && k.spread_runs_2sbc[gg.m]) {
+      threads = 128;
+      return k.spread_runs_2sbc[gg.m];
     }
     if (spread && variant == 7 && gg.d == 3 && k.spread_runs_2st[gg.m]) {
       threads = 128;
@@ -1447,6 +1490,7 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
       auto fn = spread ? k.spread_wave[t][gg.m] : k.interp_wave[t][gg.m];
       if (fn) return fn;
     }
+    if (!spread && shared_poly && k.interp_sym[t][gg.m]) return k.interp_sym[t][gg.m];
     return spread ? k.spread_static[t][gg.m] : k.interp_static[t][gg.m];
   }
   return spread ? k.spread_rt[gg.d - 1][tile ? 1 : 0] : k.interp_rt[gg.d - 1][tile ? 1 : 0];
@@ -1463,7 +1507,8 @@ void launch_interp(const cplx<R> *g, cplx<R> *f, const Engine<R> &e, hipStream_t
   if (e.nsub == 0) return;
   size_t lds;
   int threads;
-  auto fn = pick_bkernel<R>(bkernels<R>(), e.geom, false, e.interp_variant, lds, threads);
+  auto fn = pick_bkernel<R>(bkernels<R>(), e.geom, false, e.interp_variant, lds, threads,
+                            e.win.poly_shared != 0 && e.sym_window);
   hipLaunchKernelGGL(fn, dim3((unsigned)e.nsub), dim3(threads), lds, s, g, f, e.perm, e.cell, e.frac,
                      e.subs, e.geom, e.win, (cplx<R> *)nullptr);
 }

@@ -35,6 +35,31 @@ __device__ __forceinline__ float rlane_f(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// acc += x[lane J of this lane's 16-lane row] * w: the row-broadcast DPP
+// form of v_fmac_f32 (row_newbcast, gfx90a and later), so a value that every
+// lane needs is read from a register row without a readlane per use.
+template <int J>
+__device__ __forceinline__ void fmac_row_bcast(float &acc, float x, float w) {
+  asm("v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+      : "+v"(acc)
+      : "v"(x), "v"(w), "n"(J));
+}
+
+// The RW rows of one node: acc[j] += (sr, si)[row j] * (w0, w1) for the
+// lane's two pair cells, the row weights taken from lanes j of every row.
+template <int J, int RW, int SLOTS>
+__device__ __forceinline__ void rows_bcast(float2 (&acc)[RW][SLOTS][2], float br, float bi,
+                                           const float (&w0)[SLOTS], const float (&w1)[SLOTS]) {
+#pragma unroll
+  for (int sl = 0; sl < SLOTS; sl++) {
+    fmac_row_bcast<J>(acc[J][sl][0].x, br, w0[sl]);
+    fmac_row_bcast<J>(acc[J][sl][0].y, bi, w0[sl]);
+    fmac_row_bcast<J>(acc[J][sl][1].x, br, w1[sl]);
+    fmac_row_bcast<J>(acc[J][sl][1].y, bi, w1[sl]);
+  }
+  if constexpr (J + 1 < RW) rows_bcast<J + 1, RW, SLOTS>(acc, br, bi, w0, w1);
+}
+
 // More wavefronts per tile hide more latency but repeat the per-node window
 // work in each: the widest split with at most five rows groups, NW | W.
 constexpr int runs_wide_nw(int W) {
@@ -49,7 +74,11 @@ constexpr int runs_wide_nw(int W) {
 // TAB: the row weights psi0 * f_j of this wavefront's rows go through a small
 // LDS table (written by the lanes that evaluated them, read back as uniform
 // loads) instead of one readlane per row and component.
-template <int W, int NW, int G, int S, bool TAB>
+// BC: the row weights of each node are gathered once by ds_bpermute into
+// lanes 0..RW-1 of every 16-lane row and the FMAs read them with a DPP row
+// broadcast, replacing 2 RW readlanes per node (and their scalar index
+// arithmetic) by two ds_bpermute.
+template <int W, int NW, int G, int S, bool TAB, bool BC = false>
 __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restrict__ fin,
                                                          float2 *__restrict__ g,
                                                          const int *__restrict__ perm,
@@ -66,6 +95,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
   constexpr int NP = 3 * W <= 32 ? 2 : 1;  // nodes per window pass
   constexpr int NPASS = G / NP;
   static_assert(G % NP == 0 && 64 % G == 0, "groups tile the 64-node chunk");
+  static_assert(!(BC && TAB) && (!BC || RW <= 16), "one row-weight path; RW rows fit a DPP row");
   extern __shared__ unsigned char b_smem[];
   float2 *tile = reinterpret_cast<float2 *>(b_smem);
   const int4 sp = subs[blockIdx.x];
@@ -260,10 +290,45 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
           }
         }
       }
+      // BC: keys of the group's nodes and their row weights, lane q of every
+      // 16-lane row holding row r0 + q NW of the node
+      int gkey[S][G];
+      float brr[S][G], bri[S][G];
+      if constexpr (BC) {
+        const int qoff = 4 * NW * (lane & 15);
+#pragma unroll
+        for (int st = 0; st < S; st++)
+#pragma unroll
+          for (int k = 0; k < G; k++) {
+            const int p = k / NP, hk = k % NP;
+            gkey[st][k] = __builtin_amdgcn_readlane(ckey[st], base + k);
+            const int r0 = ((wv - (gkey[st][k] >> 20)) % NW + NW) % NW;
+            const int src = 4 * (hk * 32 + r0) + qoff;
+            brr[st][k] = bperm_f(src, xr[st][p]);
+            bri[st][k] = bperm_f(src, xi[st][p]);
+          }
+      }
 #pragma unroll
       for (int k = 0; k < G; k++) {
 #pragma unroll
         for (int st = 0; st < S; st++) {
+          if constexpr (BC) {
+            if (base + k < cnt[st]) {
+              const int key = gkey[st][k];
+              float a0[SLOTS], a1[SLOTS];
+#pragma unroll
+              for (int sl = 0; sl < SLOTS; sl++) {
+                a0[sl] = w0[st][k][sl];
+                a1[sl] = w1[st][k][sl];
+              }
+              if (key != cur[st]) {
+                if (cur[st] >= 0) flush(st);
+                cur[st] = key;
+              }
+              rows_bcast<0, RW, SLOTS>(acc[st], brr[st][k], bri[st][k], a0, a1);
+            }
+            continue;
+          }
           if (base + k < cnt[st]) {
             const int p = k / NP, hk = k % NP;
             const int key = __builtin_amdgcn_readlane(ckey[st], base + k);
