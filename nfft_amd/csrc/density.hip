@@ -16,10 +16,13 @@
 //    backward; training is Adam on FP32 master weights.  The output layer,
 //    the loss and the optimiser are FP32 vector code.
 //
-//    Features of node j (d = 2), 76 values padded with zeros to 80: the
+//    Features of node j (d = 2), 77 values padded with zeros to 80: the
 //    radius 2|x_j|, then for the coarse grids G = 32, 64, 128 cells per
 //    dimension log(1 + c / cbar) of the node counts c in the 5 x 5 cells
-//    around the node's cell (periodic), cbar = M / G^2.
+//    around the node's cell (periodic), cbar = M / G^2, and last the
+//    node's kernel density estimate log(v_j / mean v) with v = B B^T 1, the
+//    first Pipe-Menon denominator (one spreading and one interpolation of
+//    the plan).
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -35,7 +38,7 @@ typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kDcfFeat = 80;   // padded feature count (multiple of 16)
-constexpr int kDcfRaw = 76;    // 1 + 3 * 25
+constexpr int kDcfRaw = 77;    // 1 + 3 * 25 + 1
 constexpr int kDcfHidden = 64;
 constexpr int kDcfScales = 3;
 __constant__ int c_dcf_grid[kDcfScales] = {32, 64, 128};
@@ -134,6 +137,47 @@ __global__ void k_dcf_features(const float *__restrict__ x, long M,
   for (; q < kDcfFeat; q++) fj[q] = 0.f;
 }
 
+__global__ void k_fill_unit(float2 *__restrict__ out, long M) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < M) out[j] = float2{1.f, 0.f};
+}
+
+// FP64 partial sums of Re v over blocks of 256 nodes
+__global__ __launch_bounds__(256) void k_sum_re(const float2 *__restrict__ v, long M,
+                                                double *__restrict__ part) {
+  __shared__ double red[256];
+  const long j = (long)blockIdx.x * 256 + threadIdx.x;
+  red[threadIdx.x] = j < M ? (double)v[j].x : 0.0;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_sum_parts(const double *__restrict__ part, long nb,
+                                                   double *__restrict__ total) {
+  __shared__ double red[256];
+  double a = 0.0;
+  for (long i = threadIdx.x; i < nb; i += 256) a += part[i];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) total[0] = red[0];
+}
+
+__global__ void k_dcf_kde(const float2 *__restrict__ v, long M, const double *__restrict__ total,
+                          float *__restrict__ feat) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  const float mean = (float)(total[0] / (double)M);
+  feat[j * kDcfFeat + kDcfRaw - 1] = logf(fmaxf(v[j].x, 1e-30f) / mean);
+}
+
 static void features_device(Engine<float> *e, float *feat) {
   if (e->d != 2) family<float>::fatal("nfft: density features are defined for d = 2 plans");
   const long M = e->M;
@@ -142,12 +186,28 @@ static void features_device(Engine<float> *e, float *feat) {
   int cells = 0;
   for (int g : kDcfGridHost) cells += g * g;
   int *counts = nullptr;
+  float2 *fa = nullptr, *fb = nullptr;
+  double *part = nullptr;
+  const long nb = ceil_div(M, 256);
   NFFT_HIP_CHECK(float, hipMalloc(&counts, sizeof(int) * cells));
+  NFFT_HIP_CHECK(float, hipMalloc(&fa, sizeof(float2) * M));
+  NFFT_HIP_CHECK(float, hipMalloc(&fb, sizeof(float2) * M));
+  NFFT_HIP_CHECK(float, hipMalloc(&part, sizeof(double) * (nb + 1)));
   NFFT_HIP_CHECK(float, hipMemsetAsync(counts, 0, sizeof(int) * cells, s));
   hipLaunchKernelGGL(k_dcf_hist, dim3(nblk(M)), dim3(256), 0, s, e->xdev, M, counts);
   hipLaunchKernelGGL(k_dcf_features, dim3(nblk(M)), dim3(256), 0, s, e->xdev, M, counts, feat);
+  // kernel density estimate v = Re(B B^T 1) with the plan's own window
+  hipLaunchKernelGGL(k_fill_unit, dim3(nblk(M)), dim3(256), 0, s, fa, M);
+  e->spread_to_grid(fa);
+  e->interp_from_grid(fb);
+  hipLaunchKernelGGL(k_sum_re, dim3((unsigned)nb), dim3(256), 0, s, fb, M, part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, part, nb, part + nb);
+  hipLaunchKernelGGL(k_dcf_kde, dim3(nblk(M)), dim3(256), 0, s, fb, M, part + nb, feat);
   NFFT_HIP_CHECK(float, hipStreamSynchronize(s));
   hipFree(counts);
+  hipFree(fa);
+  hipFree(fb);
+  hipFree(part);
 }
 
 // ------------------------------------------------------------------ MLP kernels

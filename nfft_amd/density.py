@@ -4,8 +4,9 @@ reference, whose MRI applications read weights from files).
 
 * ``pipe_menon(plan, iters)``: the Pipe-Menon fixed point w <- w / (B B^T w)
   with the plan's own Kaiser-Bessel interpolation and spreading.
-* ``features(plan)``: per-node density features (radius and log node counts
-  of 5 x 5 coarse-grid neighbourhoods at three scales), a (M, 80) float32
+* ``features(plan)``: per-node density features (radius, log node counts
+  of 5 x 5 coarse-grid neighbourhoods at three scales, and the kernel
+  density estimate log(v / mean v) with v = B B^T 1), a (M, 80) float32
   device tensor.
 * ``DcfModel``: the MLP that maps features to log w, trained and evaluated by
   the library's BF16 MFMA kernels.

@@ -3,8 +3,10 @@
 Restates in numpy what nfft_amd/csrc/density.hip computes, so the GPU
 library can be checked against it:
 
-* ``features``: radius 2|x| and log(1 + c / cbar) of the node counts in the
-  5 x 5 coarse cells around each node at grids of 32, 64, 128 cells.
+* ``features``: radius 2|x|, log(1 + c / cbar) of the node counts in the
+  5 x 5 coarse cells around each node at grids of 32, 64, 128 cells, and the
+  kernel density estimate log(v / mean v), v = Re(B B^T 1) with the oracle's
+  B and B^T of the plan (N, n, m).
 * ``pipe_menon``: w <- w / Re(B B^T w) with the oracle's own B and B^T
   (oracle.stage_B / stage_BT, which use the reference's unscaled window;
   the weights agree with the library's after normalisation to mean 1).
@@ -16,15 +18,16 @@ library can be checked against it:
 The reference has no such component (its MRI applications read weights
 from files made by applications/mri/mri2d/precompute_weights.m), so these
 functions are pinned by mathematics instead: finite differences for the
-gradient, brute-force counting for the features, constant weights for a
-Cartesian node set for Pipe-Menon.
+gradient, brute-force counting for the features and a dense B built from the
+window function for the density estimate, constant weights for a Cartesian
+node set for Pipe-Menon.
 """
 import numpy as np
 
 import oracle as O
 
 NFEAT = 80
-NRAW = 76
+NRAW = 77
 H = 64
 GRIDS = (32, 64, 128)
 P_W1 = 0
@@ -50,8 +53,37 @@ def _coarse_cell(x, G):
     return np.clip(c, 0, G - 1)
 
 
-def features(x):
-    """x: (M, 2) nodes in [-1/2, 1/2).  Returns (M, 80) float64."""
+def kde(N, n, m, x):
+    """log(v / mean v), v = Re(B B^T 1): the first Pipe-Menon denominator."""
+    x = np.asarray(x, dtype=np.float64)
+    v = O.stage_B(N, n, m, x, O.stage_BT(N, n, m, x, np.ones(x.shape[0], dtype=complex))).real
+    return np.log(v / v.mean())
+
+
+def kde_bruteforce(N, n, m, x, _dm=0):
+    """The same from a dense B assembled tap by tap from the window function
+    (small M and n only).  _dm evaluates the window with cut-off m + _dm on
+    the m taps (a planted mistake for the tests)."""
+    x = np.asarray(x, dtype=np.float64)
+    M, d = x.shape
+    b = [O.kb_b(n[t] / N[t]) for t in range(d)]
+    B = np.zeros((M, int(np.prod(n))))
+    for j in range(M):
+        per = []
+        for t in range(d):
+            u = int(np.floor(n[t] * x[j, t])) - m
+            per.append([((l % n[t]), O.phi(n[t], x[j, t] - l / n[t], m + _dm, b[t]))
+                        for l in range(u, u + 2 * m + 2)])
+        for (i0, w0) in per[0]:
+            for (i1, w1) in per[1]:
+                B[j, i0 * n[1] + i1] += w0 * w1
+    v = B @ (B.T @ np.ones(M))
+    return np.log(v / v.mean())
+
+
+def features(x, N, n, m):
+    """x: (M, 2) nodes in [-1/2, 1/2) of a plan (N, n, m).  Returns (M, 80)
+    float64."""
     x = np.asarray(x, dtype=np.float32)
     M = x.shape[0]
     out = np.zeros((M, NFEAT))
@@ -67,11 +99,12 @@ def features(x):
             for b in range(-2, 3):
                 out[:, q] = np.log1p(counts[(cx + a) % G, (cy + b) % G] / cbar)
                 q += 1
+    out[:, q] = kde(N, n, m, x)
     return out
 
 
-def features_bruteforce(x):
-    """Same features by direct counting per node (for small M)."""
+def features_bruteforce(x, N, n, m):
+    """Same features by direct counting per node and a dense B (for small M)."""
     x = np.asarray(x, dtype=np.float32)
     M = x.shape[0]
     out = np.zeros((M, NFEAT))
@@ -87,6 +120,7 @@ def features_bruteforce(x):
                     c = np.count_nonzero((call[:, 0] == tx) & (call[:, 1] == ty))
                     out[j, q] = np.log1p(c * G * G / M)
                     q += 1
+    out[:, q] = kde_bruteforce(N, n, m, x)
     return out
 
 

@@ -1,7 +1,8 @@
 """Density-compensation weights on the GPU (nfft_amd/csrc/density.hip through
 the C ABI) against the CPU oracle (oracle/density.py):
 
-  features         exact counts, log1p in FP32: |diff| <= 1e-6
+  features         exact counts, log1p in FP32: |diff| <= 1e-6; the density
+                   estimate log(v / mean v), v = B B^T 1 in FP32: <= 2e-5
   Pipe-Menon       mean-normalised weights, FP32 spreading/interpolation vs
                    FP64 oracle: rel l2 <= 1e-4
   MLP forward      BF16 MFMA with FP32 accumulation vs FP64 with the same BF16
@@ -46,10 +47,15 @@ def test_features_vs_oracle(gen):
     p = make_plan([64, 64], x)
     try:
         f = D.features(p).cpu().numpy()
-        ref = OD.features(p.x.astype(np.float64))
+        ref = OD.features(p.x.astype(np.float64), [64, 64], [128, 128], 4)
     finally:
         p.finalize()
-    assert np.max(np.abs(f - ref)) <= 1e-6
+    k = OD.NRAW - 1
+    assert np.max(np.abs(f[:, :k] - ref[:, :k])) <= 1e-6
+    # density estimate: FP32 spreading and interpolation vs the FP64 oracle
+    print("kde max |diff| %.2e" % np.max(np.abs(f[:, k] - ref[:, k])))
+    assert np.max(np.abs(f[:, k] - ref[:, k])) <= 2e-5
+    assert np.all(f[:, OD.NRAW:] == 0)
 
 
 @pytest.mark.gpu

@@ -24,20 +24,37 @@ def test_bf16_round_matches_torch():
 
 def test_features_match_bruteforce():
     x = W.clustered_2d(400, seed=3)
-    fa = D.features(x)
-    fb = D.features_bruteforce(x)
+    fa = D.features(x, [16, 16], [32, 32], 4)
+    fb = D.features_bruteforce(x, [16, 16], [32, 32], 4)
     assert fa.shape == (400, D.NFEAT)
     assert np.allclose(fa, fb, rtol=0, atol=1e-12)
     assert np.all(fa[:, D.NRAW:] == 0)
+    # the density estimate follows the clustering: nodes near the centre of
+    # the cluster see more neighbours
+    r = np.hypot(x[:, 0], x[:, 1])
+    assert fa[r < 0.05, D.NRAW - 1].mean() > fa[r > 0.3, D.NRAW - 1].mean() + 1.0
+
+
+def test_kde_planted_window_mistake_is_caught():
+    """A plausible mistake in assembling B (the window evaluated with cut-off
+    m - 1) moves the density estimate far beyond the 1e-12 agreement asserted
+    above."""
+    x = W.clustered_2d(60, seed=4)
+    good = D.kde([16, 16], [32, 32], 4, x)
+    assert np.allclose(good, D.kde_bruteforce([16, 16], [32, 32], 4, x), atol=1e-12)
+    bad = D.kde_bruteforce([16, 16], [32, 32], 4, x, _dm=-1)
+    assert np.max(np.abs(good - bad)) > 1e-3
 
 
 def test_features_uniform_density_near_log2():
-    # a Cartesian node set: every coarse cell holds the mean count, so every
-    # count feature is log(1 + 1)
+    # a Cartesian node set at the cell centres of the oversampled grid: every
+    # coarse cell holds the mean count, so every count feature is log(1 + 1),
+    # and every node has the same density estimate, log(v / mean v) = 0
     g = (np.arange(128) + 0.5) / 128 - 0.5
     x = np.stack(np.meshgrid(g, g, indexing="ij"), -1).reshape(-1, 2)
-    f = D.features(x)
-    assert np.allclose(f[:, 1:D.NRAW], np.log(2.0), atol=1e-12)
+    f = D.features(x, [64, 64], [128, 128], 4)
+    assert np.allclose(f[:, 1:D.NRAW - 1], np.log(2.0), atol=1e-12)
+    assert np.allclose(f[:, D.NRAW - 1], 0.0, atol=1e-12)
 
 
 @pytest.mark.parametrize("bf16", [False])
