@@ -33,6 +33,12 @@ struct BinGeom {
   // spreading tiles go to scratch in zone order (see spread_impl.hpp):
   // needs bs_t >= m + 1 and bs_t dividing n_t
   int zoned;
+  // zoned only, built once per plan: zmap[o] = tile cell written to scratch
+  // position o; zdesc[8 k + (0, 1..3, 4..6)] = offset in the tile, core
+  // offset and size per dimension of the zone that neighbour k (digits
+  // 0, 1, 2 = left, self, right per dimension) contributes to a bin's core
+  const int *zmap;
+  const int *zdesc;
 };
 
 // Kaiser-Bessel window of one dimension, scaled by 1/phi(0) so that the B
@@ -140,6 +146,7 @@ template <typename R> struct Engine {
   long tiles_cap = 0;
   bool tile_scratch = false;
   int *scratch_i = nullptr;  // binning scratch
+  int *zone_tab = nullptr;   // geom.zmap and geom.zdesc
   long scratch_cap = 0;
   bool nodes_ready = false;
 

@@ -242,6 +242,67 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     zoned = zoned && core <= 4096;  // the zone reduction's cells per workgroup
     if (const char *z = std::getenv("NFFT_HIP_ZONES")) zoned = zoned && std::strcmp(z, "0") != 0;
     geom.zoned = zoned ? 1 : 0;
+    geom.zmap = geom.zdesc = nullptr;
+    if (zoned) {
+      // per dimension the tile splits into the part in the left neighbour's
+      // core (m cells), the own core (bs), the part in the right
+      // neighbour's core (m + 1); zones are written in base-3 order of
+      // their (z_0, ..., z_{d-1}) digits, each row-major
+      auto zstart = [&](int t, int z) { return z == 0 ? 0 : (z == 1 ? m : m + geom.bs[t]); };
+      auto zsize = [&](int t, int z) { return z == 0 ? m : (z == 1 ? geom.bs[t] : m + 1); };
+      int zc = 1;
+      for (int t = 0; t < d; t++) zc *= 3;
+      std::vector<int> tab;
+      std::vector<long> zbase(zc + 1, 0);
+      for (int zz = 0; zz < zc; zz++) {
+        int dg[kMaxDim] = {0, 0, 0, 0}, r = zz;
+        for (int t = d - 1; t >= 0; t--) {
+          dg[t] = r % 3;
+          r /= 3;
+        }
+        long cnt = 1;
+        for (int t = 0; t < d; t++) cnt *= zsize(t, dg[t]);
+        for (long q = 0; q < cnt; q++) {
+          long rem = q, src = 0, stride = 1;
+          for (int t = d - 1; t >= 0; t--) {
+            const int sz = zsize(t, dg[t]);
+            src += (zstart(t, dg[t]) + rem % sz) * stride;
+            rem /= sz;
+            stride *= geom.E[t];
+          }
+          tab.push_back((int)src);
+        }
+        zbase[zz + 1] = zbase[zz] + cnt;
+      }
+      const size_t nmap = tab.size();
+      // neighbour k with digits dg (0 left, 1 self, 2 right) contributes
+      // its zone 2 - dg_t per dimension, landing at core offset bs - m
+      // (right neighbour's left part goes to the top of the core) or 0
+      for (int k = 0; k < zc; k++) {
+        int dg[kMaxDim] = {1, 1, 1, 1}, r = k;
+        for (int t = d - 1; t >= 0; t--) {
+          dg[t] = r % 3;
+          r /= 3;
+        }
+        int zidx = 0;
+        int off[3] = {0, 0, 0}, size[3] = {1, 1, 1};
+        for (int t = 0; t < d; t++) {
+          const int z = 2 - dg[t];
+          zidx = zidx * 3 + z;
+          size[t] = zsize(t, z);
+          off[t] = dg[t] == 2 ? geom.bs[t] - m : 0;
+        }
+        tab.push_back((int)zbase[zidx]);
+        for (int t = 0; t < 3; t++) tab.push_back(off[t]);
+        for (int t = 0; t < 3; t++) tab.push_back(size[t]);
+        tab.push_back(0);
+      }
+      NFFT_HIP_CHECK(R, hipMalloc(&zone_tab, sizeof(int) * tab.size()));
+      NFFT_HIP_CHECK(R, hipMemcpy(zone_tab, tab.data(), sizeof(int) * tab.size(),
+                                  hipMemcpyHostToDevice));
+      geom.zmap = zone_tab;
+      geom.zdesc = zone_tab + nmap;
+    }
   }
   auto variant = [](const char *name, int dflt) {
     const char *v = std::getenv(name);
@@ -363,6 +424,7 @@ void Engine<R>::destroy() {
   hipFree(sp_first);
   hipFree(tiles);
   hipFree(scratch_i);
+  hipFree(zone_tab);
   for (auto &e : ev)
     if (e) hipEventDestroy(e);
   for (auto &set : log_ev)

@@ -73,12 +73,10 @@ __device__ __forceinline__ long region_to_grid(int q, const int (&lo)[D], const 
 // belongs to the left neighbour bin's core (m cells), the own core (bs) and
 // the right neighbour's core (m + 1), and stores the 3^d zones one after the
 // other (row-major over zone indices, each zone row-major inside), so the
-// tile reduction reads each contribution as one contiguous block.
-template <int D>
-__device__ __forceinline__ void zone_dims(const BinGeom &gg, int t, int z, int &start, int &size) {
-  start = z == 0 ? 0 : (z == 1 ? gg.m : gg.m + gg.bs[t]);
-  size = z == 0 ? gg.m : (z == 1 ? gg.bs[t] : gg.m + 1);
-}
+// tile reduction reads each contribution as one contiguous block.  The plan
+// builds the order once (BinGeom::zmap: tile cell of every scratch position;
+// BinGeom::zdesc: where each neighbour's zone lies in its tile and in the
+// receiving core).
 
 // Writes the block's LDS tile (values from get(q), q the LDS index) to
 // scratch, or adds it into the grid with atomics when there is no scratch.
@@ -87,40 +85,9 @@ __device__ __forceinline__ void flush_tile(Get get, cplx<R> *__restrict__ tiles,
                                            cplx<R> *__restrict__ g, const int (&lo)[D],
                                            const BinGeom &gg, int tot, int tid, int nthr) {
   if (tiles && gg.zoned) {
+    // zone order through the plan's table: coalesced writes, LDS reads
     cplx<R> *out = tiles + (long)blockIdx.x * tot;
-    int zcount = 1;
-#pragma unroll
-    for (int t = 0; t < D; t++) zcount *= 3;
-    long o = 0;
-    for (int zz = 0; zz < zcount; zz++) {
-      int st[kMaxDim], sz[kMaxDim];
-#pragma unroll
-      for (int t = 0; t < kMaxDim; t++) {
-        st[t] = 0;
-        sz[t] = 1;
-      }
-      int r = zz;
-#pragma unroll
-      for (int t = D - 1; t >= 0; t--) {
-        zone_dims<D>(gg, t, r % 3, st[t], sz[t]);
-        r /= 3;
-      }
-      int cnt = 1;
-#pragma unroll
-      for (int t = 0; t < D; t++) cnt *= sz[t];
-      for (int q = tid; q < cnt; q += nthr) {
-        int rem = q, src = 0, stride = 1;
-#pragma unroll
-        for (int t = D - 1; t >= 0; t--) {
-          const int i = rem % sz[t];
-          rem /= sz[t];
-          src += (st[t] + i) * stride;
-          stride *= gg.E[t];
-        }
-        out[o + q] = get(src);
-      }
-      o += cnt;
-    }
+    for (int o = tid; o < tot; o += nthr) out[o] = get(gg.zmap[o]);
     return;
   }
   for (int q = tid; q < tot; q += nthr) {
@@ -1217,7 +1184,8 @@ __global__ __launch_bounds__(256) void k_gather_zones(const cplx<R> *__restrict_
 #pragma unroll
   for (int t = 0; t < D; t++) tot *= gg.E[t];
   if (threadIdx.x < NZ) {
-    // neighbour offset delta_t = digit - 1; it contributes zone 2 - digit
+    // neighbour offset delta_t = digit - 1; its zone's place in the tile
+    // and in this bin's core come from the plan's table
     int dig[3] = {1, 1, 1};
     int r = threadIdx.x;
 #pragma unroll
@@ -1225,31 +1193,20 @@ __global__ __launch_bounds__(256) void k_gather_zones(const cplx<R> *__restrict_
       dig[t] = r % 3;
       r /= 3;
     }
-    int nbl = 0, zidx = 0;
+    int nbl = 0;
 #pragma unroll
     for (int t = 0; t < D; t++) {
-      const int nb = (bc[t] + dig[t] - 1 + gg.nb[t]) % gg.nb[t];
+      int nb = bc[t] + dig[t] - 1;
+      nb += nb < 0 ? gg.nb[t] : (nb >= gg.nb[t] ? -gg.nb[t] : 0);
       nbl = nbl * gg.nb[t] + nb;
-      const int z = 2 - dig[t];
-      int st, sz;
-      zone_dims<D>(gg, t, z, st, sz);
-      zidx = zidx * 3 + z;
-      s_size[threadIdx.x][t] = sz;
-      s_off[threadIdx.x][t] = dig[t] == 2 ? gg.bs[t] - gg.m : 0;
     }
-    // offset of zone zidx: sum of the sizes of the zones before it
-    long base = 0;
-    for (int q = 0; q < zidx; q++) {
-      int rq = q, c = 1;
-      for (int t = D - 1; t >= 0; t--) {
-        int st, sz;
-        zone_dims<D>(gg, t, rq % 3, st, sz);
-        rq /= 3;
-        c *= sz;
-      }
-      base += c;
+    const int *zd = gg.zdesc + 8 * threadIdx.x;
+    s_base[threadIdx.x] = zd[0];
+#pragma unroll
+    for (int t = 0; t < D; t++) {
+      s_off[threadIdx.x][t] = zd[1 + t];
+      s_size[threadIdx.x][t] = zd[4 + t];
     }
-    s_base[threadIdx.x] = base;
     s_first[threadIdx.x] = sp_first[nbl];
     s_cnt[threadIdx.x] = sp_count[nbl];
   }
