@@ -139,6 +139,7 @@ template <typename R> struct Engine {
   int4 *sp_subs = nullptr;
   long sp_nsub = 0, sp_subs_cap = 0;
   int *sp_first = nullptr, *sp_count = nullptr;
+  int *bin_order = nullptr;  // zone reduction's bin order (inside sp_first's allocation)
   long bin_arrays_cap = 0;
   int spread_max_per_sub = 2048;
   // spreading tiles of all work items (no HBM atomics); null: atomic flush

@@ -178,6 +178,53 @@ __global__ void k_fill_subs(const int *__restrict__ hist, const int *__restrict_
   }
 }
 
+// Order of the bins for the zone reduction: bins whose 3^d neighbourhoods
+// hold the most spreading work items first (16 buckets by log2 of that
+// count), so that the few long-running workgroups of dense regions start at
+// the beginning of the launch instead of ending it.
+constexpr int kOrderBuckets = 16;
+__global__ void k_bin_weight(const int *__restrict__ count_of, BinGeom gg, long nbins,
+                             int *__restrict__ bucket_of, int *__restrict__ bucket_hist) {
+  long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  int bc[kMaxDim] = {0, 0, 0, 0};
+  long r = b;
+  for (int t = gg.d - 1; t >= 0; t--) {
+    bc[t] = (int)(r % gg.nb[t]);
+    r /= gg.nb[t];
+  }
+  int zc = 1;
+  for (int t = 0; t < gg.d; t++) zc *= 3;
+  long pairs = 0;
+  for (int k = 0; k < zc; k++) {
+    int rk = k;
+    long nbl = 0;
+    int dig[kMaxDim] = {1, 1, 1, 1};
+    for (int t = gg.d - 1; t >= 0; t--) {
+      dig[t] = rk % 3;
+      rk /= 3;
+    }
+    for (int t = 0; t < gg.d; t++) {
+      int v = bc[t] + dig[t] - 1;
+      v += v < 0 ? gg.nb[t] : (v >= gg.nb[t] ? -gg.nb[t] : 0);
+      nbl = nbl * gg.nb[t] + v;
+    }
+    pairs += count_of[nbl];
+  }
+  int lg = 0;
+  while (lg < kOrderBuckets - 1 && (2l << lg) <= pairs + 1) lg++;
+  const int bucket = kOrderBuckets - 1 - lg;  // heaviest first
+  bucket_of[b] = bucket;
+  atomicAdd(&bucket_hist[bucket], 1);
+}
+
+__global__ void k_bin_order(const int *__restrict__ bucket_of, long nbins,
+                            int *__restrict__ bucket_next, int *__restrict__ order) {
+  long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  order[atomicAdd(&bucket_next[bucket_of[b]], 1)] = (int)b;
+}
+
 __global__ void k_fill_zero_int(int *__restrict__ p, long count) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = 0;
@@ -206,7 +253,7 @@ void bin_nodes(Engine<R> &e) {
   // scratch layout, each array 256-byte aligned: khist, kstart (nkeys ints),
   // tile_sum, tile_off (ntiles ints), hist, bin_start, nsub_of, sub_start
   // (nbins ints each), key_of, rank_of (M ints each), cell_tmp (d M ints),
-  // frac_tmp (d M reals), sums (4 ints)
+  // frac_tmp (d M reals), sums (64 ints: scan totals, bin-order buckets)
   auto al = [](long bytes) { return (bytes + 255) / 256 * 256; };
   const long b_keys = al(nkeys * (long)sizeof(int));
   const long b_tiles = al(ntiles * (long)sizeof(int));
@@ -281,11 +328,26 @@ void bin_nodes(Engine<R> &e) {
   make_list(e.max_per_sub, sub_start, nsub_of, &e.subs, &e.subs_cap, &e.nsub);
   if (nbins > e.bin_arrays_cap) {
     if (e.sp_first) NFFT_HIP_CHECK(R, hipFree(e.sp_first));
-    NFFT_HIP_CHECK(R, hipMalloc(&e.sp_first, 2 * nbins * sizeof(int)));
+    NFFT_HIP_CHECK(R, hipMalloc(&e.sp_first, 3 * nbins * sizeof(int)));
     e.bin_arrays_cap = nbins;
   }
   e.sp_count = e.sp_first + nbins;
+  e.bin_order = e.sp_first + 2 * nbins;
   make_list(e.spread_max_per_sub, e.sp_first, e.sp_count, &e.sp_subs, &e.sp_subs_cap, &e.sp_nsub);
+  // heavy bins first for the zone reduction (the interpolation list's
+  // temporaries are free again: bucket per bin, bucket histogram and offsets)
+  if (gg.zoned) {
+    int *bucket_of = sub_start;
+    int *bhist = sums + 2;
+    int *boff = sums + 2 + kOrderBuckets;
+    launch_fill_zero_int(bhist, kOrderBuckets, s);
+    hipLaunchKernelGGL(k_bin_weight, dim3((unsigned)ceil_div(nbins, 256)), dim3(256), 0, s,
+                       e.sp_count, gg, nbins, bucket_of, bhist);
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, bhist, boff,
+                       (long)kOrderBuckets, sums + 2 + 2 * kOrderBuckets);
+    hipLaunchKernelGGL(k_bin_order, dim3((unsigned)ceil_div(nbins, 256)), dim3(256), 0, s,
+                       bucket_of, nbins, boff, e.bin_order);
+  }
   NFFT_HIP_CHECK(R, hipGetLastError());
 }
 

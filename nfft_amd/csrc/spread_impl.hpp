@@ -1163,7 +1163,8 @@ template <typename R, int D, int CPT>
 __global__ __launch_bounds__(256) void k_gather_zones(const cplx<R> *__restrict__ tiles,
                                                       const int *__restrict__ sp_first,
                                                       const int *__restrict__ sp_count,
-                                                      cplx<R> *__restrict__ g, BinGeom gg) {
+                                                      cplx<R> *__restrict__ g, BinGeom gg,
+                                                      const int *__restrict__ order) {
   constexpr int NZ = D == 1 ? 3 : (D == 2 ? 9 : 27);
   constexpr int BATCH = CPT >= 16 ? 1 : 16 / CPT;
   __shared__ long s_base[NZ];   // offset of the zone inside a tile
@@ -1173,7 +1174,7 @@ __global__ __launch_bounds__(256) void k_gather_zones(const cplx<R> *__restrict_
   __shared__ int s_pz[kZonePairs];
   int bc[3] = {0, 0, 0};
   {
-    int r = blockIdx.x;
+    int r = order[blockIdx.x];  // heavy neighbourhoods first
 #pragma unroll
     for (int t = D - 1; t >= 0; t--) {
       bc[t] = r % gg.nb[t];
@@ -1298,7 +1299,8 @@ void launch_gather_tiles(const Engine<R> &e, cplx<R> *g, hipStream_t s) {
     long core = 1;
     for (int t = 0; t < e.geom.d; t++) core *= e.geom.bs[t];
     if (core <= 4096) {
-      using F = void (*)(const cplx<R> *, const int *, const int *, cplx<R> *, BinGeom);
+      using F = void (*)(const cplx<R> *, const int *, const int *, cplx<R> *, BinGeom,
+                         const int *);
       F fz;
       if (e.geom.d == 3)
         fz = core <= 512 ? k_gather_zones<R, 3, 2> : (core <= 1024 ? k_gather_zones<R, 3, 4>
@@ -1308,7 +1310,7 @@ void launch_gather_tiles(const Engine<R> &e, cplx<R> *g, hipStream_t s) {
       else
         fz = core <= 1024 ? k_gather_zones<R, 1, 4> : k_gather_zones<R, 1, 16>;
       hipLaunchKernelGGL(fz, dim3((unsigned)nbins), dim3(256), 0, s, e.tiles, e.sp_first,
-                         e.sp_count, g, e.geom);
+                         e.sp_count, g, e.geom, e.bin_order);
       return;
     }
   }
