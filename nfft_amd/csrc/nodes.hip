@@ -174,7 +174,8 @@ __global__ void k_fill_subs(const int *__restrict__ hist, const int *__restrict_
   for (int s = 0; s * max_per_sub < cnt; s++) {
     int first = bin_start[b] + s * max_per_sub;
     int c = min(max_per_sub, cnt - s * max_per_sub);
-    subs[s0 + s] = make_int4((int)b, first, c, 0);
+    // .w: the item's place in bin order (its spreading tile's slot)
+    subs[s0 + s] = make_int4((int)b, first, c, s0 + s);
   }
 }
 
@@ -225,6 +226,27 @@ __global__ void k_bin_order(const int *__restrict__ bucket_of, long nbins,
   order[atomicAdd(&bucket_next[bucket_of[b]], 1)] = (int)b;
 }
 
+// Launch order of the work items: largest node counts first (buckets by
+// log2 of the count), so that long items do not end the launch.
+__global__ void k_sub_weight(const int4 *__restrict__ subs, long nsub, int *__restrict__ bucket_of,
+                             int *__restrict__ bucket_hist) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nsub) return;
+  const int c = subs[i].z;
+  int lg = 0;
+  while (lg < kOrderBuckets - 1 && (2 << lg) <= c) lg++;
+  const int bucket = kOrderBuckets - 1 - lg;
+  bucket_of[i] = bucket;
+  atomicAdd(&bucket_hist[bucket], 1);
+}
+
+__global__ void k_sub_order(const int4 *__restrict__ in, const int *__restrict__ bucket_of,
+                            long nsub, int *__restrict__ bucket_next, int4 *__restrict__ out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nsub) return;
+  out[atomicAdd(&bucket_next[bucket_of[i]], 1)] = in[i];
+}
+
 __global__ void k_fill_zero_int(int *__restrict__ p, long count) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = 0;
@@ -253,7 +275,7 @@ void bin_nodes(Engine<R> &e) {
   // scratch layout, each array 256-byte aligned: khist, kstart (nkeys ints),
   // tile_sum, tile_off (ntiles ints), hist, bin_start, nsub_of, sub_start
   // (nbins ints each), key_of, rank_of (M ints each), cell_tmp (d M ints),
-  // frac_tmp (d M reals), sums (64 ints: scan totals, bin-order buckets)
+  // frac_tmp (d M reals), sums (256 ints: scan totals, ordering buckets)
   auto al = [](long bytes) { return (bytes + 255) / 256 * 256; };
   const long b_keys = al(nkeys * (long)sizeof(int));
   const long b_tiles = al(ntiles * (long)sizeof(int));
@@ -261,7 +283,7 @@ void bin_nodes(Engine<R> &e) {
   const long b_M = al(M * (long)sizeof(int));
   const long b_cell = al((long)gg.d * M * (long)sizeof(int));
   const long b_frac = al((long)gg.d * M * (long)sizeof(R));
-  const long need = 2 * b_keys + 2 * b_tiles + 4 * b_bins + 2 * b_M + b_cell + b_frac + 256;
+  const long need = 2 * b_keys + 2 * b_tiles + 4 * b_bins + 2 * b_M + b_cell + b_frac + 1024;
   if (need > e.scratch_cap) {
     if (e.scratch_i) NFFT_HIP_CHECK(R, hipFree(e.scratch_i));
     NFFT_HIP_CHECK(R, hipMalloc(&e.scratch_i, need));
@@ -285,7 +307,7 @@ void bin_nodes(Engine<R> &e) {
   int *rank_of = reinterpret_cast<int *>(take(b_M));
   int *cell_tmp = reinterpret_cast<int *>(take(b_cell));
   R *frac_tmp = reinterpret_cast<R *>(take(b_frac));
-  int *sums = reinterpret_cast<int *>(take(256));
+  int *sums = reinterpret_cast<int *>(take(1024));
 
   hipStream_t s = e.stream;
   launch_fill_zero_int(khist, nkeys, s);
@@ -315,15 +337,29 @@ void bin_nodes(Engine<R> &e) {
     int total = 0;
     NFFT_HIP_CHECK(R, hipMemcpyAsync(&total, sums + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     NFFT_HIP_CHECK(R, hipStreamSynchronize(s));
-    if (total > *cap) {
+    // the list, then the same list in bin order (the ordering's input)
+    if (2L * total > *cap) {
       if (*subs) NFFT_HIP_CHECK(R, hipFree(*subs));
-      NFFT_HIP_CHECK(R, hipMalloc(subs, (size_t)total * sizeof(int4)));
-      *cap = total;
+      NFFT_HIP_CHECK(R, hipMalloc(subs, (size_t)2 * total * sizeof(int4)));
+      *cap = 2L * total;
     }
     *nsub = total;
-    if (total > 0)
+    if (total > 0) {
+      int4 *in_bin_order = *subs + total;
       hipLaunchKernelGGL(k_fill_subs, dim3((unsigned)ceil_div(nbins, 256)), dim3(256), 0, s, hist,
-                         bin_start, first_of, nbins, max_per_sub, *subs);
+                         bin_start, first_of, nbins, max_per_sub, in_bin_order);
+      // largest items first; key_of (M ints, total <= M) is free again
+      int *bucket_of = key_of;
+      int *bhist = sums + 2 + 3 * kOrderBuckets;
+      int *boff = bhist + kOrderBuckets;
+      launch_fill_zero_int(bhist, kOrderBuckets, s);
+      hipLaunchKernelGGL(k_sub_weight, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, s,
+                         in_bin_order, (long)total, bucket_of, bhist);
+      hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, bhist, boff,
+                         (long)kOrderBuckets, boff + kOrderBuckets);
+      hipLaunchKernelGGL(k_sub_order, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, s,
+                         in_bin_order, bucket_of, (long)total, boff, *subs);
+    }
   };
   make_list(e.max_per_sub, sub_start, nsub_of, &e.subs, &e.subs_cap, &e.nsub);
   if (nbins > e.bin_arrays_cap) {

@@ -79,20 +79,23 @@ __device__ __forceinline__ long region_to_grid(int q, const int (&lo)[D], const 
 // receiving core).
 
 // Writes the block's LDS tile (values from get(q), q the LDS index) to
-// scratch, or adds it into the grid with atomics when there is no scratch.
+// scratch slot `slot` (the work item's place in bin order, which the tile
+// reductions rely on; items are launched largest first), or adds it into the
+// grid with atomics when there is no scratch.
 template <typename R, int D, typename Get>
 __device__ __forceinline__ void flush_tile(Get get, cplx<R> *__restrict__ tiles,
                                            cplx<R> *__restrict__ g, const int (&lo)[D],
-                                           const BinGeom &gg, int tot, int tid, int nthr) {
+                                           const BinGeom &gg, int tot, long slot, int tid,
+                                           int nthr) {
   if (tiles && gg.zoned) {
     // zone order through the plan's table: coalesced writes, LDS reads
-    cplx<R> *out = tiles + (long)blockIdx.x * tot;
+    cplx<R> *out = tiles + slot * tot;
     for (int o = tid; o < tot; o += nthr) out[o] = get(gg.zmap[o]);
     return;
   }
   for (int q = tid; q < tot; q += nthr) {
     const cplx<R> v = get(q);
-    if (tiles) tiles[(long)blockIdx.x * tot + q] = v;
+    if (tiles) tiles[slot * tot + q] = v;
     else if (v.x != R(0) || v.y != R(0)) {
       const long gi = region_to_grid<D>(q, lo, gg);
       unsafeAtomicAdd(&g[gi].x, v.x);
@@ -414,7 +417,7 @@ __global__ __launch_bounds__(256) void k_spread_tile(const cplx<R> *__restrict__
     }
   }
   __syncthreads();
-  flush_tile<R, D>([&](int q) { return acc.load(tile + q); }, tiles, g, lo, gg, tot, threadIdx.x,
+  flush_tile<R, D>([&](int q) { return acc.load(tile + q); }, tiles, g, lo, gg, tot, sp.w, threadIdx.x,
                    blockDim.x);
 }
 
@@ -567,7 +570,7 @@ __global__ __launch_bounds__(256) void k_bwave_tile(const cplx<R> *__restrict__ 
   if constexpr (SPREAD) {
     __syncthreads();
     flush_tile<R, D>([&](int q) { return acc.load(atile + q); }, tiles, dst, lo, gg, tot,
-                     threadIdx.x, blockDim.x);
+                     sp.w, threadIdx.x, blockDim.x);
   }
 }
 
@@ -692,7 +695,7 @@ __global__ __launch_bounds__(64) void k_spread_wave_priv(const cplx<R> *__restri
     cf = nf;
   }
   __syncthreads();
-  flush_tile<R, D>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, lane, 64);
+  flush_tile<R, D>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, sp.w, lane, 64);
 }
 
 // ================================================== row-partitioned B^T, d = 3
@@ -828,7 +831,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_rows(const cplx<R> *__restri
     cf = nf;
   }
   __syncthreads();
-  flush_tile<R, 3>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, threadIdx.x, blockDim.x);
+  flush_tile<R, 3>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, sp.w, threadIdx.x, blockDim.x);
 }
 
 }  // namespace nfft_hip
@@ -1051,7 +1054,7 @@ __global__ __launch_bounds__(256) void k_spread_rt(const cplx<R> *__restrict__ f
   if constexpr (TILE) {
     __syncthreads();
     flush_tile<R, D>([&](int q) { return acc.load(tile + q); }, tiles, g, lo, gg, tot,
-                     threadIdx.x, blockDim.x);
+                     sp.w, threadIdx.x, blockDim.x);
   }
 }
 

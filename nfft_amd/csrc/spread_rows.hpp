@@ -169,7 +169,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_rows2(const cplx<R> *__restr
     cf = nf;
   }
   __syncthreads();
-  flush_tile<R, 3>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, threadIdx.x, blockDim.x);
+  flush_tile<R, 3>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, sp.w, threadIdx.x, blockDim.x);
 }
 
 }  // namespace nfft_hip

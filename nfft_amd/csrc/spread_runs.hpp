@@ -371,7 +371,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
   for (int st = 0; st < S; st++)
     if (cur[st] >= 0) flush(st);
   __syncthreads();
-  flush_tile<float, 3>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, threadIdx.x,
+  flush_tile<float, 3>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, sp.w, threadIdx.x,
                        blockDim.x);
 }
 
