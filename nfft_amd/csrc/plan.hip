@@ -220,8 +220,14 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     geom.nb[t] = t < d ? (int)ceil_div(n[t], bs[t]) : 1;
     geom.E[t] = t < d ? bs[t] + 2 * m + 1 : 1;
   }
-  max_per_sub = d == 3 ? 1024 : 512;
-  spread_max_per_sub = d == 3 ? 2048 : 1024;
+  // nodes per work item; larger 3-D spreading items write fewer tiles for
+  // the reduction (measured on the 3-D kooshball: 2048 -> 4096 halves the
+  // tile reduction at equal spreading time, larger items end the launch late)
+  max_per_sub = d == 3 ? 2048 : 512;
+  spread_max_per_sub = d == 3 ? 4096 : 1024;
+  if (const char *v = std::getenv("NFFT_HIP_ITEM")) max_per_sub = std::max(64, std::atoi(v));
+  if (const char *v = std::getenv("NFFT_HIP_SPREAD_ITEM"))
+    spread_max_per_sub = std::max(64, std::atoi(v));
   // B^T writes its LDS tiles to scratch and reduces them without atomics
   // when the tile fits LDS and never wraps onto itself (E_t <= n_t)
   {
