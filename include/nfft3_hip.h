@@ -23,7 +23,7 @@ extern "C"
 /* An instantiated hipGraph of batched transforms (see P##graph_trafo). */
 typedef struct nfft_hip_graph_s nfft_hip_graph;
 
-#define NFFT_HIP_DECLARE_EXT(P, PLAN)                                        \
+#define NFFT_HIP_DECLARE_EXT(P, PLAN, REAL)                                  \
   /* f_dev <- A f_hat_dev (same math as P##trafo) */                          \
   void P##trafo_device(PLAN *ths, const void *f_hat_dev, void *f_dev);       \
   /* f_hat_dev <- A^H f_dev (same math as P##adjoint) */                      \
@@ -52,6 +52,14 @@ typedef struct nfft_hip_graph_s nfft_hip_graph;
   /* Number of B-stage work items (sub-problems) built by the last           \
    * precompute, for diagnostics. */                                         \
   long P##num_subproblems(PLAN *ths);                                       \
+  /* Gridding reconstruction f_hat = A^H (w .* f), w real (NULL: plain     \
+   * adjoint); the weighted adjoint of the reference's MRI applications     \
+   * (applications/mri/mri2d/reconstruct_data_gridding.c:54-68, which      \
+   * scales f by the weights of weights.dat and calls nfft_adjoint).  f is  \
+   * not modified.  The host form reads ths->f and w, writes ths->f_hat. */ \
+  void P##gridding_device(PLAN *ths, const void *f_dev, const REAL *w_dev,  \
+                          void *f_hat_dev);                                  \
+  void P##gridding(PLAN *ths, const REAL *w);                                \
   /* nrhs right-hand sides sharing the plan's nodes (e.g. MRI coils):       \
    * vector r of f_hat_dev starts at r N_total, of f_dev at r M_total. */    \
   void P##trafo_batch_device(PLAN *ths, const void *f_hat_dev, void *f_dev, \
@@ -70,8 +78,8 @@ typedef struct nfft_hip_graph_s nfft_hip_graph;
                                    const void **f_dev, void **f_hat_dev,     \
                                    int nrhs);
 
-NFFT_HIP_DECLARE_EXT(nfftf_, nfftf_plan)
-NFFT_HIP_DECLARE_EXT(nfft_, nfft_plan)
+NFFT_HIP_DECLARE_EXT(nfftf_, nfftf_plan, float)
+NFFT_HIP_DECLARE_EXT(nfft_, nfft_plan, double)
 
 /* Replay a captured graph (enqueued, no wait), wait for it, release it. */
 void nfft_hip_graph_launch(nfft_hip_graph *graph);

@@ -296,6 +296,20 @@ class Plan:
                 "adjoint_B", "adjoint_Bred", "adjoint_FFT", "adjoint_D"]
         return dict(zip(keys, list(out)))
 
+    def gridding_device(self, f_ptr, w_ptr, f_hat_ptr):
+        """f_hat = A^H (w .* f) on device buffers; w_ptr 0: plain adjoint."""
+        self._fn("gridding_device")(self.pp, ctypes.c_void_p(f_ptr), ctypes.c_void_p(w_ptr or None),
+                                    ctypes.c_void_p(f_hat_ptr))
+
+    def gridding(self, w=None):
+        """f_hat = A^H (w .* f) with the plan's host f and f_hat (f unchanged)."""
+        if w is None:
+            self._fn("gridding")(self.pp, None)
+            return
+        w = np.ascontiguousarray(w, dtype=self.rdtype)
+        assert w.size == self.M
+        self._fn("gridding")(self.pp, w.ctypes.data_as(ctypes.c_void_p))
+
     def trafo_batch_device(self, f_hat_ptr, f_ptr, nrhs):
         self._fn("trafo_batch_device")(self.pp, ctypes.c_void_p(f_hat_ptr), ctypes.c_void_p(f_ptr),
                                        int(nrhs))

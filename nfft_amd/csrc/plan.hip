@@ -692,6 +692,14 @@ static void set_mv(nfft_plan *p) {
   p->mv_adjoint = reinterpret_cast<void (*)(void *)>(nfft_adjoint);
 }
 
+// out = w .* f (complex values times real weights)
+template <typename R>
+__global__ void k_weight_mul(const cplx<R> *__restrict__ f, const R *__restrict__ w,
+                             cplx<R> *__restrict__ out, long M) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < M) out[j] = cplx<R>{w[j] * f[j].x, w[j] * f[j].y};
+}
+
 template <typename R, typename PLAN>
 struct PlanApi {
   static Engine<R> *eng(const PLAN *p) {
@@ -936,6 +944,62 @@ struct PlanApi {
     if (e->timing) store_times(p, e, true);
   }
 
+  // Gridding reconstruction f_hat = A^H (w .* f), the weighted adjoint of the
+  // reference's MRI applications (applications/mri/mri2d/
+  // reconstruct_data_gridding.c:54-68 scales f by the weights of weights.dat
+  // and calls nfft_adjoint).  The weighted values go to the plan's device f
+  // buffer; the caller's f is left as it is.  w == NULL: plain adjoint.
+  static void gridding_device(PLAN *p, const void *f, const R *w, void *fh) {
+    if (!w) {
+      adjoint_device(p, f, fh);
+      return;
+    }
+    clear_launch_error();
+    Engine<R> *e = eng(p);
+    if (e->direct_only) {
+      if (!e->nodes_ready) precompute(p);
+    } else {
+      ensure_nodes(p, e);
+    }
+    const long M = p->M_total;
+    if (M > 0)
+      hipLaunchKernelGGL(k_weight_mul<R>, dim3((unsigned)ceil_div(M, 256)), dim3(256), 0,
+                         e->stream, (const cplx<R> *)f, w, e->f, M);
+    if (e->direct_only) {
+      e->adjoint_direct(e->f, (cplx<R> *)fh);
+      return;
+    }
+    e->adjoint(e->f, (cplx<R> *)fh);
+    if (e->timing) store_times(p, e, true);
+  }
+
+  // host arrays: f_hat = A^H (w .* f) with the plan's f and f_hat
+  static void gridding(PLAN *p, const R *w) {
+    clear_launch_error();
+    Engine<R> *e = eng(p);
+    const long M = p->M_total;
+    std::vector<cplx<R>> fw((size_t)std::max(0L, M));
+    const R *f = reinterpret_cast<const R *>(p->f);
+    for (long j = 0; j < M; j++) {
+      const R wj = w ? w[j] : R(1);
+      fw[j] = cplx<R>{wj * f[2 * j], wj * f[2 * j + 1]};
+    }
+    if (e->direct_only) {
+      if (!e->nodes_ready) precompute(p);
+    } else {
+      ensure_nodes(p, e);
+    }
+    if (M > 0)
+      NFFT_HIP_CHECK(R, hipMemcpyAsync(e->f, fw.data(), sizeof(cplx<R>) * (size_t)M,
+                                       hipMemcpyHostToDevice, e->stream));
+    if (e->direct_only) e->adjoint_direct(e->f, e->fhat);
+    else e->adjoint(e->f, e->fhat);
+    NFFT_HIP_CHECK(R, hipMemcpyAsync(p->f_hat, e->fhat, sizeof(cplx<R>) * (size_t)p->N_total,
+                                     hipMemcpyDeviceToHost, e->stream));
+    NFFT_HIP_CHECK(R, hipStreamSynchronize(e->stream));
+    if (!e->direct_only) store_times(p, e, true);
+  }
+
   // nrhs transforms sharing the plan's nodes: vectors r of f_hat_dev / f_dev
   // at offsets r N_total / r M_total, enqueued back to back
   static void trafo_batch_device(PLAN *p, const void *fh, void *f, int nrhs) {
@@ -1116,6 +1180,10 @@ using nfft_hip::PlanApi;
   void P##timing_begin(PLAN *p) { PlanApi<R, PLAN>::eng(p)->timing_begin(); }                 \
   void P##timing_end(PLAN *p, double *out) { PlanApi<R, PLAN>::eng(p)->timing_end(out); }     \
   long P##num_subproblems(PLAN *p) { return PlanApi<R, PLAN>::num_subproblems(p); }           \
+  void P##gridding_device(PLAN *p, const void *f, const R *w, void *fh) {                     \
+    PlanApi<R, PLAN>::gridding_device(p, f, w, fh);                                           \
+  }                                                                                           \
+  void P##gridding(PLAN *p, const R *w) { PlanApi<R, PLAN>::gridding(p, w); }                 \
   void P##trafo_batch_device(PLAN *p, const void *fh, void *f, int nrhs) {                    \
     PlanApi<R, PLAN>::trafo_batch_device(p, fh, f, nrhs);                                     \
   }                                                                                           \
