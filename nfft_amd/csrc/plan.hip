@@ -315,6 +315,8 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     if (!v) return dflt;
     return std::strcmp(v, "runs1w3s") == 0  ? 9
            : std::strcmp(v, "runs2sbc") == 0 ? 10
+           : std::strcmp(v, "runs1w2sbc") == 0 ? 11
+           : std::strcmp(v, "runs1w3sbc") == 0 ? 12
            : std::strcmp(v, "runs1w2s") == 0 ? 8
            : std::strcmp(v, "runs2st") == 0 ? 7
            : std::strcmp(v, "runs2s") == 0 ? 6
@@ -538,7 +540,7 @@ void Engine<R>::autotune() {
     // float candidates accumulate in plain FP32 (the fixed-point LDS
     // variants are left out); double ones use ds_add_f64
     std::vector<int> cand;
-    if (sizeof(R) == 4 && d == 3) cand = {10, 6, 8, 9, 7, 4, 5, 3};
+    if (sizeof(R) == 4 && d == 3) cand = {10, 11, 6, 8, 9, 7, 4, 5, 3};
     else if (sizeof(R) == 8 && d >= 2) cand = {1, 0};
     float best = 1e30f;
     int pick = spread_variant;

@@ -1339,6 +1339,8 @@ template <typename R> struct BKernelTable {
   fn spread_runs_2s[kMaxStaticM + 1];         // same, two node streams per wavefront
   fn spread_runs_2st[kMaxStaticM + 1];        // same, row weights through LDS
   fn spread_runs_2sbc[kMaxStaticM + 1];       // same, row weights by DPP row broadcast
+  fn spread_runs_1w2sbc[kMaxStaticM + 1];     // one wavefront per tile, two streams, DPP
+  fn spread_runs_1w3sbc[kMaxStaticM + 1];     // one wavefront per tile, three streams, DPP
   fn spread_runs_1w2s[kMaxStaticM + 1];       // one wavefront per tile, two streams
   fn spread_runs_1w3s[kMaxStaticM + 1];       // one wavefront per tile, three streams
   fn interp_rt[kMaxDim][2];  // [d-1][tile]
@@ -1362,6 +1364,10 @@ void bk_fill_one(BKernelTable<R> &k) {
     k.spread_runs_2s[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false>;
     k.spread_runs_2st[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, true>;
     k.spread_runs_2sbc[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false, true>;
+    if constexpr (2 * M_ + 2 <= 16) {  // all rows' weights in one 16-lane DPP row
+      k.spread_runs_1w2sbc[M_] = k_spread_runs<2 * M_ + 2, 1, 8, 2, false, true>;
+      k.spread_runs_1w3sbc[M_] = k_spread_runs<2 * M_ + 2, 1, 8, 3, false, true>;
+    }
     k.spread_runs_1w2s[M_] = k_spread_runs<2 * M_ + 2, 1, 8, 2, false>;
     k.spread_runs_1w3s[M_] = k_spread_runs<2 * M_ + 2, 1, 8, 3, false>;
   }
@@ -1416,6 +1422,10 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
     if (spread && (variant == 8 || variant == 9) && gg.d == 3 && k.spread_runs_1w2s[gg.m]) {
       threads = 64;
       return variant == 8 ? k.spread_runs_1w2s[gg.m] : k.spread_runs_1w3s[gg.m];
+    }
+    if (spread && (variant == 11 || variant == 12) && gg.d == 3 && k.spread_runs_1w2sbc[gg.m]) {
+      threads = 64;
+      return variant == 11 ? k.spread_runs_1w2sbc[gg.m] : k.spread_runs_1w3sbc[gg.m];
     }
     if (spread && variant == 10 && gg.d == 3 && k.spread_runs_2sbc[gg.m]) {
       threads = 128;
