@@ -42,7 +42,6 @@ constexpr int kDcfRaw = 77;    // 1 + 3 * 25 + 1
 constexpr int kDcfHidden = 64;
 constexpr int kDcfScales = 3;
 __constant__ int c_dcf_grid[kDcfScales] = {32, 64, 128};
-constexpr int kDcfGridHost[kDcfScales] = {32, 64, 128};
 
 // flat FP32 parameter layout; weights [in][out]
 constexpr int P_W1 = 0;
@@ -75,13 +74,56 @@ __global__ void k_pm_update(float *__restrict__ w, const float2 *__restrict__ v,
 
 static unsigned nblk(long n, int t = 256) { return (unsigned)std::max<long>(1, ceil_div(n, t)); }
 
+// Rows per chunk of learned-weight inference (features and MLP workspace are
+// sized for one chunk, not for M).
+constexpr long kDcfChunk = 65536;
+
+// The helpers' device scratch, carved from the plan's aux buffer: optional
+// features of one chunk (kDcfChunk x 80 floats), coarse-grid counts, two
+// complex node vectors and the FP64 partial sums of a mean.
+struct DcfScratch {
+  float *feat = nullptr;
+  int *counts = nullptr;
+  float2 *fa = nullptr, *fb = nullptr;
+  double *part = nullptr;
+};
+constexpr int kDcfCells = 32 * 32 + 64 * 64 + 128 * 128;
+
+static DcfScratch dcf_scratch(Engine<float> *e, bool with_feat) {
+  const long M = e->M;
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t b_feat = with_feat ? al(sizeof(float) * std::min(M, kDcfChunk) * kDcfFeat) : 0;
+  const size_t b_counts = al(sizeof(int) * kDcfCells);
+  const size_t b_vec = al(sizeof(float2) * M);
+  const size_t b_part = al(sizeof(double) * (ceil_div(M, 256) + 1));
+  const size_t need = b_feat + b_counts + 2 * b_vec + b_part;
+  if (need > e->aux_cap) {
+    NFFT_HIP_CHECK(float, hipStreamSynchronize(e->stream));
+    hipFree(e->aux);
+    e->aux = nullptr;
+    NFFT_HIP_CHECK(float, hipMalloc(&e->aux, need));
+    e->aux_cap = need;
+  }
+  char *p = e->aux;
+  DcfScratch sc;
+  if (with_feat) sc.feat = reinterpret_cast<float *>(p);
+  p += b_feat;
+  sc.counts = reinterpret_cast<int *>(p);
+  p += b_counts;
+  sc.fa = reinterpret_cast<float2 *>(p);
+  p += b_vec;
+  sc.fb = reinterpret_cast<float2 *>(p);
+  p += b_vec;
+  sc.part = reinterpret_cast<double *>(p);
+  return sc;
+}
+
 static void pipe_menon_device(Engine<float> *e, float *w, int iters) {
   const long M = e->M;
   if (M == 0) return;
   hipStream_t s = e->stream;
-  float2 *fa = nullptr, *fb = nullptr;
-  NFFT_HIP_CHECK(float, hipMalloc(&fa, sizeof(float2) * M));
-  NFFT_HIP_CHECK(float, hipMalloc(&fb, sizeof(float2) * M));
+  const DcfScratch sc = dcf_scratch(e, false);
+  float2 *fa = sc.fa, *fb = sc.fb;
   for (int it = 0; it < iters; it++) {
     hipLaunchKernelGGL(k_real_to_cplx, dim3(nblk(M)), dim3(256), 0, s, w, fa, M);
     e->spread_to_grid(fa);
@@ -89,8 +131,6 @@ static void pipe_menon_device(Engine<float> *e, float *w, int iters) {
     hipLaunchKernelGGL(k_pm_update, dim3(nblk(M)), dim3(256), 0, s, w, fb, M);
   }
   NFFT_HIP_CHECK(float, hipStreamSynchronize(s));
-  hipFree(fa);
-  hipFree(fb);
 }
 
 // ------------------------------------------------------------------ features
@@ -112,12 +152,14 @@ __global__ void k_dcf_hist(const float *__restrict__ x, long M, int *__restrict_
   }
 }
 
-__global__ void k_dcf_features(const float *__restrict__ x, long M,
+// rows j0 .. j0+n-1 of the features into feat[0 .. n-1]
+__global__ void k_dcf_features(const float *__restrict__ x, long M, long j0, long n,
                                const int *__restrict__ counts, float *__restrict__ feat) {
-  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= M) return;
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const long j = j0 + r;
   const float x0 = x[2 * j], x1 = x[2 * j + 1];
-  float *fj = feat + j * kDcfFeat;
+  float *fj = feat + r * kDcfFeat;
   fj[0] = 2.f * sqrtf(x0 * x0 + x1 * x1);
   int off = 0, q = 1;
 #pragma unroll
@@ -170,44 +212,49 @@ __global__ __launch_bounds__(256) void k_sum_parts(const double *__restrict__ pa
   if (threadIdx.x == 0) total[0] = red[0];
 }
 
-__global__ void k_dcf_kde(const float2 *__restrict__ v, long M, const double *__restrict__ total,
-                          float *__restrict__ feat) {
-  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= M) return;
+__global__ void k_dcf_kde(const float2 *__restrict__ v, long M, long j0, long n,
+                          const double *__restrict__ total, float *__restrict__ feat) {
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
   const float mean = (float)(total[0] / (double)M);
-  feat[j * kDcfFeat + kDcfRaw - 1] = logf(fmaxf(v[j].x, 1e-30f) / mean);
+  feat[r * kDcfFeat + kDcfRaw - 1] = logf(fmaxf(v[j0 + r].x, 1e-30f) / mean);
 }
 
+// The parts of the features shared by all rows, enqueued on the plan's
+// stream: coarse-grid counts, and the density estimate v = Re(B B^T 1) with
+// its FP64 sum at part[nb]
+static void features_common(Engine<float> *e, const DcfScratch &sc) {
+  const long M = e->M;
+  hipStream_t s = e->stream;
+  const long nb = ceil_div(M, 256);
+  NFFT_HIP_CHECK(float, hipMemsetAsync(sc.counts, 0, sizeof(int) * kDcfCells, s));
+  hipLaunchKernelGGL(k_dcf_hist, dim3(nblk(M)), dim3(256), 0, s, e->xdev, M, sc.counts);
+  hipLaunchKernelGGL(k_fill_unit, dim3(nblk(M)), dim3(256), 0, s, sc.fa, M);
+  e->spread_to_grid(sc.fa);
+  e->interp_from_grid(sc.fb);
+  hipLaunchKernelGGL(k_sum_re, dim3((unsigned)nb), dim3(256), 0, s, sc.fb, M, sc.part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, sc.part, nb, sc.part + nb);
+}
+
+// feature rows j0 .. j0+n-1 into feat, after features_common
+static void features_rows(Engine<float> *e, const DcfScratch &sc, long j0, long n, float *feat) {
+  const long M = e->M;
+  hipStream_t s = e->stream;
+  hipLaunchKernelGGL(k_dcf_features, dim3(nblk(n)), dim3(256), 0, s, e->xdev, M, j0, n, sc.counts,
+                     feat);
+  hipLaunchKernelGGL(k_dcf_kde, dim3(nblk(n)), dim3(256), 0, s, sc.fb, M, j0, n,
+                     sc.part + ceil_div(M, 256), feat);
+}
+
+// all M feature rows into the caller's feat
 static void features_device(Engine<float> *e, float *feat) {
   if (e->d != 2) family<float>::fatal("nfft: density features are defined for d = 2 plans");
   const long M = e->M;
   if (M == 0) return;
-  hipStream_t s = e->stream;
-  int cells = 0;
-  for (int g : kDcfGridHost) cells += g * g;
-  int *counts = nullptr;
-  float2 *fa = nullptr, *fb = nullptr;
-  double *part = nullptr;
-  const long nb = ceil_div(M, 256);
-  NFFT_HIP_CHECK(float, hipMalloc(&counts, sizeof(int) * cells));
-  NFFT_HIP_CHECK(float, hipMalloc(&fa, sizeof(float2) * M));
-  NFFT_HIP_CHECK(float, hipMalloc(&fb, sizeof(float2) * M));
-  NFFT_HIP_CHECK(float, hipMalloc(&part, sizeof(double) * (nb + 1)));
-  NFFT_HIP_CHECK(float, hipMemsetAsync(counts, 0, sizeof(int) * cells, s));
-  hipLaunchKernelGGL(k_dcf_hist, dim3(nblk(M)), dim3(256), 0, s, e->xdev, M, counts);
-  hipLaunchKernelGGL(k_dcf_features, dim3(nblk(M)), dim3(256), 0, s, e->xdev, M, counts, feat);
-  // kernel density estimate v = Re(B B^T 1) with the plan's own window
-  hipLaunchKernelGGL(k_fill_unit, dim3(nblk(M)), dim3(256), 0, s, fa, M);
-  e->spread_to_grid(fa);
-  e->interp_from_grid(fb);
-  hipLaunchKernelGGL(k_sum_re, dim3((unsigned)nb), dim3(256), 0, s, fb, M, part);
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, part, nb, part + nb);
-  hipLaunchKernelGGL(k_dcf_kde, dim3(nblk(M)), dim3(256), 0, s, fb, M, part + nb, feat);
-  NFFT_HIP_CHECK(float, hipStreamSynchronize(s));
-  hipFree(counts);
-  hipFree(fa);
-  hipFree(fb);
-  hipFree(part);
+  const DcfScratch sc = dcf_scratch(e, false);
+  features_common(e, sc);
+  features_rows(e, sc, 0, M, feat);
+  NFFT_HIP_CHECK(float, hipStreamSynchronize(e->stream));
 }
 
 // ------------------------------------------------------------------ MLP kernels
@@ -574,6 +621,9 @@ nfft_dcf_model *nfft_dcf_model_create(unsigned seed) {
   NFFT_HIP_CHECK(float, hipMemset(m.grad, 0, sizeof(float) * P_COUNT));
   NFFT_HIP_CHECK(float, hipMemset(m.m, 0, sizeof(float) * P_COUNT));
   NFFT_HIP_CHECK(float, hipMemset(m.v, 0, sizeof(float) * P_COUNT));
+  // workspace of one inference chunk (training batches up to that size reuse
+  // it), so that inference allocates nothing after creation
+  m.ensure((int)kDcfChunk);
   // He initialisation of the weights, zero biases
   std::vector<float> p(P_COUNT, 0.f);
   std::mt19937 rng(seed);
@@ -646,9 +696,14 @@ float nfft_dcf_model_train_step(nfft_dcf_model *h, const float *feat_dev, const 
 }
 
 void nfft_dcf_model_predict(nfft_dcf_model *h, const float *feat_dev, int rows, float *out_dev) {
-  nfft_hip::clear_launch_error();
+  using namespace nfft_hip;
+  clear_launch_error();
   DcfModel &m = h->m;
-  m.forward(feat_dev, nullptr, rows, out_dev);
+  // in chunks, so the workspace stays at kDcfChunk rows
+  for (long r0 = 0; r0 < rows; r0 += kDcfChunk) {
+    const int n = (int)std::min<long>(kDcfChunk, rows - r0);
+    m.forward(feat_dev + r0 * kDcfFeat, nullptr, n, out_dev + r0);
+  }
   NFFT_HIP_CHECK(float, hipStreamSynchronize(m.stream));
 }
 
@@ -656,13 +711,27 @@ void nfftf_density_learned_device(nfftf_plan *ths, nfft_dcf_model *h, float *w_d
   using namespace nfft_hip;
   const long M = ths->M_total;
   if (M == 0) return;
-  float *feat = nullptr;
-  NFFT_HIP_CHECK(float, hipMalloc(&feat, sizeof(float) * M * kDcfFeat));
-  nfftf_dcf_features_device(ths, feat);
-  nfft_dcf_model_predict(h, feat, (int)M, w_dev);
-  hipLaunchKernelGGL(k_exp, dim3(nblk(M)), dim3(256), 0, h->m.stream, w_dev, w_dev, M);
-  NFFT_HIP_CHECK(float, hipStreamSynchronize(h->m.stream));
-  hipFree(feat);
+  clear_launch_error();
+  auto *e = engine_of_mv<float>(ths);
+  if (!e) family<float>::fatal("nfft: density weights need a plan of this library");
+  ensure_nodes_of_mv<float>(ths);
+  if (e->d != 2) family<float>::fatal("nfft: density features are defined for d = 2 plans");
+  // features chunk by chunk in the plan's scratch, each chunk's inference
+  // right behind it on the plan's stream (the model's kernels follow the
+  // plan's work; calls on one model must not overlap)
+  const DcfScratch sc = dcf_scratch(e, true);
+  features_common(e, sc);
+  DcfModel &m = h->m;
+  const hipStream_t own = m.stream;
+  m.stream = e->stream;
+  for (long j0 = 0; j0 < M; j0 += kDcfChunk) {
+    const long n = std::min(kDcfChunk, M - j0);
+    features_rows(e, sc, j0, n, sc.feat);
+    m.forward(sc.feat, nullptr, (int)n, w_dev + j0);
+  }
+  hipLaunchKernelGGL(k_exp, dim3(nblk(M)), dim3(256), 0, e->stream, w_dev, w_dev, M);
+  m.stream = own;
+  NFFT_HIP_CHECK(float, hipStreamSynchronize(e->stream));
 }
 
 }  // extern "C"

@@ -148,6 +148,10 @@ template <typename R> struct Engine {
   bool tile_scratch = false;
   int *scratch_i = nullptr;  // binning scratch
   int *zone_tab = nullptr;   // geom.zmap and geom.zdesc
+  // device scratch of the density-weight helpers (features, Pipe-Menon),
+  // grown on demand and kept with the plan so repeated calls allocate nothing
+  char *aux = nullptr;
+  size_t aux_cap = 0;
   long scratch_cap = 0;
   bool nodes_ready = false;
 

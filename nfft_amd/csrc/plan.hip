@@ -225,6 +225,14 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   // tile reduction at equal spreading time, larger items end the launch late)
   max_per_sub = d == 3 ? 2048 : 512;
   spread_max_per_sub = d == 3 ? 4096 : 1024;
+  if (d == 2) {
+    // 2-D spreading items: about 1024 of them for a plan's nodes, 256..1024
+    // nodes each (few large items leave dense regions to a handful of
+    // wavefronts; many small ones multiply the tiles to reduce)
+    long per = 256;
+    while (per < 1024 && per * 1024 < M) per *= 2;
+    spread_max_per_sub = (int)per;
+  }
   if (const char *v = std::getenv("NFFT_HIP_ITEM")) max_per_sub = std::max(64, std::atoi(v));
   if (const char *v = std::getenv("NFFT_HIP_SPREAD_ITEM"))
     spread_max_per_sub = std::max(64, std::atoi(v));
@@ -317,6 +325,7 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
            : std::strcmp(v, "runs2sbc") == 0 ? 10
            : std::strcmp(v, "runs1w2sbc") == 0 ? 11
            : std::strcmp(v, "runs1w3sbc") == 0 ? 12
+           : std::strcmp(v, "runs2d") == 0 ? 14
            : std::strcmp(v, "runs1w2s") == 0 ? 8
            : std::strcmp(v, "runs2st") == 0 ? 7
            : std::strcmp(v, "runs2s") == 0 ? 6
@@ -331,9 +340,10 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   interp_variant = variant("NFFT_HIP_INTERP", 0);
   if (const char *v = std::getenv("NFFT_HIP_SYMWIN")) sym_window = std::atoi(v) != 0;
   // float: plain FP32 read-modify-write by wavefronts that own disjoint
-  // cells (row-partitioned with same-cell runs summed in registers in 3-D,
-  // private tiles in 2-D); double: wavefronts sharing a tile with ds_add_f64
-  spread_variant = variant("NFFT_HIP_SPREAD", sizeof(R) == 4 ? (d == 3 ? 10 : 2) : 1);
+  // cells, same-cell runs summed in registers (3-D: two wavefronts per tile
+  // split by row parity; 2-D: one wavefront per tile; 1-D: private tiles);
+  // double: wavefronts sharing a tile with ds_add_f64
+  spread_variant = variant("NFFT_HIP_SPREAD", sizeof(R) == 4 ? (d == 3 ? 10 : (d == 2 ? 14 : 2)) : 1);
 
   // FFT: twiddles per dimension, then the pruned passes of both directions
   const int cap = sizeof(R) == 4 ? 4096 : 2048;  // elements per workgroup
@@ -433,6 +443,7 @@ void Engine<R>::destroy() {
   hipFree(tiles);
   hipFree(scratch_i);
   hipFree(zone_tab);
+  hipFree(aux);
   for (auto &e : ev)
     if (e) hipEventDestroy(e);
   for (auto &set : log_ev)
@@ -541,6 +552,7 @@ void Engine<R>::autotune() {
     // variants are left out); double ones use ds_add_f64
     std::vector<int> cand;
     if (sizeof(R) == 4 && d == 3) cand = {10, 11, 6, 8, 9, 7, 4, 5, 3};
+    else if (sizeof(R) == 4 && d == 2) cand = {14, 1, 0, 2};
     else if (sizeof(R) == 8 && d >= 2) cand = {1, 0};
     float best = 1e30f;
     int pick = spread_variant;

@@ -837,6 +837,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_rows(const cplx<R> *__restri
 }  // namespace nfft_hip
 #include "spread_rows.hpp"
 #include "spread_runs.hpp"
+#include "spread_runs2d.hpp"
 namespace nfft_hip {
 
 // ==================================================================== runtime m
@@ -1309,7 +1310,8 @@ void launch_gather_tiles(const Engine<R> &e, cplx<R> *g, hipStream_t s) {
         fz = core <= 512 ? k_gather_zones<R, 3, 2> : (core <= 1024 ? k_gather_zones<R, 3, 4>
                                                                     : k_gather_zones<R, 3, 16>);
       else if (e.geom.d == 2)
-        fz = core <= 1024 ? k_gather_zones<R, 2, 4> : k_gather_zones<R, 2, 16>;
+        fz = core <= 256 ? k_gather_zones<R, 2, 1>
+                         : (core <= 1024 ? k_gather_zones<R, 2, 4> : k_gather_zones<R, 2, 16>);
       else
         fz = core <= 1024 ? k_gather_zones<R, 1, 4> : k_gather_zones<R, 1, 16>;
       hipLaunchKernelGGL(fz, dim3((unsigned)nbins), dim3(256), 0, s, e.tiles, e.sp_first,
@@ -1340,6 +1342,7 @@ template <typename R> struct BKernelTable {
   fn spread_runs_2st[kMaxStaticM + 1];        // same, row weights through LDS
   fn spread_runs_2sbc[kMaxStaticM + 1];       // same, row weights by DPP row broadcast
   fn spread_runs_1w2sbc[kMaxStaticM + 1];     // one wavefront per tile, two streams, DPP
+  fn spread_runs2d[kMaxStaticM + 1];          // d = 2, float: run-merging, one wavefront
   fn spread_runs_1w3sbc[kMaxStaticM + 1];     // one wavefront per tile, three streams, DPP
   fn spread_runs_1w2s[kMaxStaticM + 1];       // one wavefront per tile, two streams
   fn spread_runs_1w3s[kMaxStaticM + 1];       // one wavefront per tile, three streams
@@ -1358,6 +1361,8 @@ void bk_fill_one(BKernelTable<R> &k) {
     k.spread_priv[D - 1][M_] = k_spread_wave_priv<R, D, 2 * M_ + 2>;
   }
   if constexpr (D == 3 && D * (2 * M_ + 2) <= 64) k.spread_rows[M_] = k_spread_rows2<R, 2 * M_ + 2, 2>;
+  if constexpr (std::is_same_v<R, float> && D == 2)
+    k.spread_runs2d[M_] = k_spread_runs2d<2 * M_ + 2, ((2 * M_ + 2) * (M_ + 1) > 64 ? 4 : 8), 2>;
   if constexpr (std::is_same_v<R, float> && D == 3 && D * (2 * M_ + 2) <= 64) {
     k.spread_runs[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 1, false>;
     k.spread_runs_wide[M_] = k_spread_runs<2 * M_ + 2, runs_wide_nw(2 * M_ + 2), 8, 1, false>;
@@ -1422,6 +1427,10 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
     if (spread && (variant == 8 || variant == 9) && gg.d == 3 && k.spread_runs_1w2s[gg.m]) {
       threads = 64;
       return variant == 8 ? k.spread_runs_1w2s[gg.m] : k.spread_runs_1w3s[gg.m];
+    }
+    if (spread && variant == 14 && gg.d == 2 && k.spread_runs2d[gg.m]) {
+      threads = 64;
+      return k.spread_runs2d[gg.m];
     }
     if (spread && (variant == 11 || variant == 12) && gg.d == 3 && k.spread_runs_1w2sbc[gg.m]) {
       threads = 64;

@@ -1,0 +1,236 @@
+// B^T for d = 2 in FP32 with same-cell runs summed in registers.
+//
+// One wavefront per work item and LDS tile (E0 x E1 cells).  A lane owns one
+// tap of dimension 0 and a pair of adjacent taps of dimension 1 of the
+// current node, so its read-modify-writes of the tile never collide with
+// another lane's (W (W/2) pairs over SLOTS passes of the 64 lanes).  Nodes
+// arrive sorted by cell; consecutive nodes of one cell form a run whose
+// contributions the lanes sum in registers and add to the tile once.  Two
+// node streams (the two halves of the work item) interleave node by node for
+// two independent dependency chains; a wavefront's LDS operations complete
+// in program order, so the streams' updates of shared cells do not race.
+//
+// Window values: G nodes at a time, each lane evaluates one tap (dimension
+// wt, tap ww) of one node as a per-tap polynomial, two nodes per pass when
+// 2 W <= 32.  Lanes of dimension 0 fold the node value in (psi0 f_j, complex),
+// and every lane fetches its three factors by ds_bpermute.
+#pragma once
+
+#include "engine.hpp"
+#include "window.hpp"
+
+namespace nfft_hip {
+
+template <int W, int G, int S>
+__global__ __launch_bounds__(64) void k_spread_runs2d(const float2 *__restrict__ fin,
+                                                      float2 *__restrict__ g,
+                                                      const int *__restrict__ perm,
+                                                      const int *__restrict__ cell,
+                                                      const float *__restrict__ frac,
+                                                      const int4 *__restrict__ subs,
+                                                      BinGeom gg, WinParams<float> wp,
+                                                      float2 *__restrict__ tiles) {
+  static_assert(W % 2 == 0 && 2 * W <= 64, "W = 2m+2 taps");
+  constexpr int HW = W / 2;
+  constexpr int PAIRS = W * HW;
+  constexpr int SLOTS = (PAIRS + 63) / 64;
+  constexpr int NP = 2 * W <= 32 ? 2 : 1;  // nodes per window pass
+  constexpr int NPASS = G / NP;
+  static_assert(G % NP == 0 && 64 % G == 0, "groups tile the 64-node chunk");
+  extern __shared__ unsigned char b_smem[];
+  float2 *tile = reinterpret_cast<float2 *>(b_smem);
+  const int4 sp = subs[blockIdx.x];
+  int lo[2];
+  bin_corner<2>(sp.x, gg, lo);
+  const int E1 = gg.E[1];
+  const int tot = gg.E[0] * E1;
+  const int lane = threadIdx.x;
+  for (int q = lane; q < tot; q += 64) tile[q] = float2{0.f, 0.f};
+
+  // (dimension-0 tap, dimension-1 pair) slots of this lane
+  int off[SLOTS];
+  bool pv[SLOTS];
+  int adr_u[NP][SLOTS], adr_v0[NP][SLOTS], adr_v1[NP][SLOTS];
+#pragma unroll
+  for (int sl = 0; sl < SLOTS; sl++) {
+    const int q = lane + 64 * sl;
+    pv[sl] = q < PAIRS;
+    const int pa = pv[sl] ? q / HW : 0;
+    const int pb = pv[sl] ? 2 * (q - (q / HW) * HW) : 0;
+    off[sl] = pa * E1 + pb;
+#pragma unroll
+    for (int h = 0; h < NP; h++) {
+      adr_u[h][sl] = 4 * (h * 32 + pa);
+      adr_v0[h][sl] = 4 * (h * 32 + W + pb);
+      adr_v1[h][sl] = 4 * (h * 32 + W + pb + 1);
+    }
+  }
+  // window tap evaluated by this lane in a pass: node half h, dimension wt, tap ww
+  const int h = NP == 2 ? lane >> 5 : 0;
+  const int li = NP == 2 ? lane & 31 : lane;
+  const int wt = li / W < 2 ? li / W : 1;
+  const int ww = li - (li / W) * W;
+  float coef[kPolyCoef];
+  {
+    const float *pc = wp.poly + (wt * W + ww) * kPolyCoef;
+#pragma unroll
+    for (int k = 0; k < kPolyCoef; k++) coef[k] = pc[k];
+  }
+
+  int sb[S], sn[S];
+  {
+    const int part = (sp.z + S - 1) / S;
+#pragma unroll
+    for (int st = 0; st < S; st++) {
+      sb[st] = min(sp.z, st * part);
+      sn[st] = min(sp.z, (st + 1) * part) - sb[st];
+    }
+  }
+  float2 acc[S][SLOTS][2];
+#pragma unroll
+  for (int st = 0; st < S; st++)
+#pragma unroll
+    for (int sl = 0; sl < SLOTS; sl++) acc[st][sl][0] = acc[st][sl][1] = float2{0.f, 0.f};
+  int cur[S];
+#pragma unroll
+  for (int st = 0; st < S; st++) cur[st] = -1;
+  __syncthreads();
+  auto flush = [&](int st) {
+    const int corner = cur[st];
+    // all reads, then all writes: the slots are distinct cells
+    float2 v[SLOTS][2];
+#pragma unroll
+    for (int sl = 0; sl < SLOTS; sl++)
+      if (pv[sl]) {
+        v[sl][0] = tile[corner + off[sl]];
+        v[sl][1] = tile[corner + off[sl] + 1];
+      }
+#pragma unroll
+    for (int sl = 0; sl < SLOTS; sl++) {
+      if (pv[sl]) {
+        tile[corner + off[sl]] = float2{v[sl][0].x + acc[st][sl][0].x, v[sl][0].y + acc[st][sl][0].y};
+        tile[corner + off[sl] + 1] =
+            float2{v[sl][1].x + acc[st][sl][1].x, v[sl][1].y + acc[st][sl][1].y};
+      }
+      acc[st][sl][0] = acc[st][sl][1] = float2{0.f, 0.f};
+    }
+  };
+
+  // chunk of 64 nodes per stream, one per lane (a valid node for every lane;
+  // lanes past the stream's end repeat its first node and are never used)
+  auto pos = [&](int st, int first) {
+    const int i = first + lane;
+    return (long)sp.y + (sn[st] > 0 ? sb[st] + (i < sn[st] ? i : 0) : 0);
+  };
+  int ckey[S];
+  float cfr[S][2];
+  float2 cf[S];
+  int nc[S][2];
+  float nfr[S][2];
+  float2 nf[S];
+  int nperm[S], nnperm[S];
+  auto load_raw = [&](int st, int first, int pidx) {
+    const long p = pos(st, first);
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      nc[st][t] = cell[t * gg.M + p];
+      nfr[st][t] = frac[t * gg.M + p];
+    }
+    nf[st] = fin[pidx];
+  };
+  auto convert = [&](int st) {
+    ckey[st] = (nc[st][0] - lo[0]) * E1 + (nc[st][1] - lo[1]);
+#pragma unroll
+    for (int t = 0; t < 2; t++) cfr[st][t] = nfr[st][t] - 0.5f;
+    cf[st] = nf[st];
+  };
+  int longest = 0;
+#pragma unroll
+  for (int st = 0; st < S; st++) {
+    load_raw(st, 0, perm[pos(st, 0)]);
+    nperm[st] = perm[pos(st, 64)];
+    nnperm[st] = 0;
+    convert(st);
+    longest = max(longest, sn[st]);
+  }
+  for (int first = 0; first < longest; first += 64) {
+    int cnt[S];
+    int cmax = 0;
+#pragma unroll
+    for (int st = 0; st < S; st++) {
+      if (first + 64 < sn[st]) {
+        load_raw(st, first + 64, nperm[st]);
+        nnperm[st] = perm[pos(st, first + 128)];
+      }
+      cnt[st] = max(0, min(64, sn[st] - first));
+      cmax = max(cmax, cnt[st]);
+    }
+    for (int base = 0; base < cmax; base += G) {
+      // factors of nodes base .. base+G-1 of every stream
+      float ur[S][G][SLOTS], ui[S][G][SLOTS], v0[S][G][SLOTS], v1[S][G][SLOTS];
+#pragma unroll
+      for (int st = 0; st < S; st++) {
+#pragma unroll
+        for (int p = 0; p < NPASS; p++) {
+          const int src = 4 * (base + NP * p + h);
+          const float f0 = bperm_f(src, cfr[st][0]);
+          const float f1 = bperm_f(src, cfr[st][1]);
+          const float fx = bperm_f(src, cf[st].x);
+          const float fy = bperm_f(src, cf[st].y);
+          const float tt = wt == 0 ? f0 : f1;
+          float v = coef[kPolyDeg];
+#pragma unroll
+          for (int k = kPolyDeg - 1; k >= 0; k--) v = fmaf(v, tt, coef[k]);
+          // dimension-0 lanes carry psi0 f_j, dimension-1 lanes psi1
+          const float vr = wt == 0 ? v * fx : v;
+          const float vi = wt == 0 ? v * fy : v;
+#pragma unroll
+          for (int hk = 0; hk < NP; hk++) {
+            const int k = NP * p + hk;
+#pragma unroll
+            for (int sl = 0; sl < SLOTS; sl++) {
+              ur[st][k][sl] = bperm_f(adr_u[hk][sl], vr);
+              ui[st][k][sl] = bperm_f(adr_u[hk][sl], vi);
+              v0[st][k][sl] = bperm_f(adr_v0[hk][sl], v);
+              v1[st][k][sl] = bperm_f(adr_v1[hk][sl], v);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < G; k++) {
+#pragma unroll
+        for (int st = 0; st < S; st++) {
+          if (base + k < cnt[st]) {
+            const int key = __builtin_amdgcn_readlane(ckey[st], base + k);
+            if (key != cur[st]) {
+              if (cur[st] >= 0) flush(st);
+              cur[st] = key;
+            }
+#pragma unroll
+            for (int sl = 0; sl < SLOTS; sl++) {
+              acc[st][sl][0].x = fmaf(ur[st][k][sl], v0[st][k][sl], acc[st][sl][0].x);
+              acc[st][sl][0].y = fmaf(ui[st][k][sl], v0[st][k][sl], acc[st][sl][0].y);
+              acc[st][sl][1].x = fmaf(ur[st][k][sl], v1[st][k][sl], acc[st][sl][1].x);
+              acc[st][sl][1].y = fmaf(ui[st][k][sl], v1[st][k][sl], acc[st][sl][1].y);
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int st = 0; st < S; st++) {
+      if (first + 64 < sn[st]) {
+        convert(st);
+        nperm[st] = nnperm[st];
+      }
+    }
+  }
+#pragma unroll
+  for (int st = 0; st < S; st++)
+    if (cur[st] >= 0) flush(st);
+  __syncthreads();
+  flush_tile<float, 2>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, sp.w, lane, 64);
+}
+
+}  // namespace nfft_hip
