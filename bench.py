@@ -31,6 +31,8 @@ Also reported:
   cpu_baseline  the FP64 CPU oracle on a bounded sample of the same workload
                 (full-grid D + FFT, B on 2^18 of the nodes, scaled to M),
                 single-threaded on the host
+  batched       32 coil right-hand sides on one 2-D plan and 64 plans of
+                uneven size, eager and as hipGraph replays
   cgnr          the metric's second part on held-out 2-D trajectories
                 (N = 256^2, sigma 2, m = 4, FP32; golden-angle radial and
                 variable-density spiral sets; phantom of random ellipses):
@@ -39,7 +41,7 @@ Also reported:
                 computation, to normal-equation residual 1e-6 (noise-free
                 data) and to an image-error target (1% complex noise)
 
-python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-cgnr]
+python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-cgnr] [--no-batch]
 """
 import argparse
 import json
@@ -214,6 +216,94 @@ def cgnr_leg(args):
     return out
 
 
+def batch_leg(args):
+    """Batched FP32 transforms (the two batched workloads): (a) 32 coil
+    right-hand sides on one 2-D N=256^2 plan (sigma 2, m 4) of 403 x 512
+    golden-angle radial nodes, trafo + adjoint of all 32, eager
+    (trafo_batch_device / adjoint_batch_device) and as one hipGraph replay;
+    (b) 64 plans of N=128^2 with M log-uniform in [1e4, 1e6] (iid uniform
+    nodes), trafo + adjoint of every plan as two hipGraph replays.  Inputs
+    resident on the device; nodes/s counts every right-hand side's nodes."""
+    import torch
+    import nfft_amd as nf
+    from nfft_amd import workloads as W
+    flags = (nf.PRE_PHI_HUT | nf.PRE_PSI | nf.MALLOC_X | nf.MALLOC_F_HAT | nf.MALLOC_F
+             | nf.FFTW_INIT)
+    out = {}
+
+    def timed(fn, reps=10):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    # (a) 32 right-hand sides, one node set
+    x = W.golden_radial(403, 512)
+    p = nf.Plan([256, 256], x.shape[0], n=[512, 512], m=4, flags=flags, precision="single")
+    p.x[:] = x
+    p.precompute()
+    M, NN, R = p.M, 256 * 256, 32
+    stream = torch.cuda.current_stream()
+    p.set_stream(stream.cuda_stream)
+    fh = torch.randn(R * NN, dtype=torch.complex64, device="cuda")
+    f = torch.randn(R * M, dtype=torch.complex64, device="cuda")
+    fo = torch.empty_like(f)
+    ho = torch.empty_like(fh)
+
+    def eager():
+        p.trafo_batch_device(fh.data_ptr(), fo.data_ptr(), R)
+        p.adjoint_batch_device(f.data_ptr(), ho.data_ptr(), R)
+    t_eager = timed(eager)
+    gt = nf.Graph([p], [fh.data_ptr()], [fo.data_ptr()], nrhs=R)
+    ga = nf.Graph([p], [f.data_ptr()], [ho.data_ptr()], nrhs=R, adjoint=True)
+
+    def graphs():
+        gt.launch()
+        ga.launch()
+        ga.synchronize()
+    t_graph = timed(graphs)
+    gt.destroy()
+    ga.destroy()
+    p.finalize()
+    out["coils_32"] = {"M": M, "nrhs": R, "eager_ms": t_eager * 1e3, "graph_ms": t_graph * 1e3,
+                       "nodes_per_s_graph": R * M / t_graph}
+    log("batch: 32 coils, M=%d: eager %.2f ms, graph %.2f ms per trafo+adjoint of all"
+        % (M, t_eager * 1e3, t_graph * 1e3))
+
+    # (b) 64 plans of uneven node count
+    r = np.random.default_rng(4)
+    Ms = np.round(10 ** r.uniform(4, 6, 64)).astype(int)
+    plans, ins, outs, ains, aouts = [], [], [], [], []
+    for i, Mi in enumerate(Ms):
+        q = nf.Plan([128, 128], int(Mi), n=[256, 256], m=4, flags=flags, precision="single")
+        q.x[:] = W.uniform(int(Mi), 2, seed=100 + i)
+        q.precompute()
+        plans.append(q)
+        ins.append(torch.randn(128 * 128, dtype=torch.complex64, device="cuda"))
+        outs.append(torch.empty(int(Mi), dtype=torch.complex64, device="cuda"))
+        ains.append(torch.randn(int(Mi), dtype=torch.complex64, device="cuda"))
+        aouts.append(torch.empty(128 * 128, dtype=torch.complex64, device="cuda"))
+    gt = nf.Graph(plans, [t.data_ptr() for t in ins], [t.data_ptr() for t in outs])
+    ga = nf.Graph(plans, [t.data_ptr() for t in ains], [t.data_ptr() for t in aouts], adjoint=True)
+
+    def graphs64():
+        gt.launch()
+        ga.launch()
+        ga.synchronize()
+    t64 = timed(graphs64)
+    gt.destroy()
+    ga.destroy()
+    for q in plans:
+        q.finalize()
+    out["plans_64"] = {"M_total": int(Ms.sum()), "graph_ms": t64 * 1e3,
+                       "nodes_per_s": float(Ms.sum()) / t64}
+    log("batch: 64 plans, %d nodes: graph %.2f ms per trafo+adjoint of all" % (Ms.sum(), t64 * 1e3))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,6 +313,7 @@ def main():
     ap.add_argument("--samples", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cgnr", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cgnr-max-iter", type=int, default=1000)
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -392,6 +483,8 @@ def main():
         "cpu_baseline": cpu,
     }
     p.finalize()
+    if not args.no_batch:
+        out["batched"] = batch_leg(args)
     if not args.no_cgnr:
         out["cgnr"] = cgnr_leg(args)
     print(json.dumps(out))

@@ -39,6 +39,9 @@ struct BinGeom {
   // 0, 1, 2 = left, self, right per dimension) contributes to a bin's core
   const int *zmap;
   const int *zdesc;
+  // batched launches (blockIdx.y = right-hand side): per right-hand side
+  // offsets of the grid, the node vector and the scratch tiles; 0 otherwise
+  long bs_g, bs_f, bs_t;
 };
 
 // Kaiser-Bessel window of one dimension, scaled by 1/phi(0) so that the B
@@ -102,6 +105,9 @@ template <typename R> struct FftPass {
   int radix[20];
   const cplx<R> *twiddle;
   int twiddle_in_lds;  // copy the twiddle table to LDS (when it fits 64 KB)
+  // batched launches (blockIdx.y = right-hand side): offsets of the grid and
+  // of the coefficient vectors per right-hand side
+  long bs_g, bs_c;
 };
 
 template <typename R> struct Engine {
@@ -188,6 +194,16 @@ template <typename R> struct Engine {
   void spread_to_grid(const cplx<R> *f_dev);
   void interp_from_grid(cplx<R> *f_dev);
   void adjoint(const cplx<R> *f_dev, cplx<R> *fhat_dev);
+  // nrhs right-hand sides with the same nodes in batched launches (grid y =
+  // right-hand side) when the plan's kernels support it, else one by one
+  bool batchable() const;
+  void reserve_batch(int nrhs);
+  void trafo_batch(const cplx<R> *fhat_dev, cplx<R> *f_dev, int nrhs);
+  void adjoint_batch(const cplx<R> *f_dev, cplx<R> *fhat_dev, int nrhs);
+  cplx<R> *bgrid = nullptr;   // batched grids
+  cplx<R> *btiles = nullptr;  // batched spreading tiles
+  long bgrid_cap = 0, btiles_cap = 0;  // in complex values
+  int batch_chunk(int nrhs) const;
   void trafo_direct(const cplx<R> *fhat_dev, cplx<R> *f_dev);
   void adjoint_direct(const cplx<R> *f_dev, cplx<R> *fhat_dev);
   void stage_times(double out[4]);
@@ -214,15 +230,22 @@ void launch_deconv_to_grid(const cplx<R> *fhat, cplx<R> *g, const R *phi_inv, in
 template <typename R>
 void launch_deconv_from_grid(const cplx<R> *g, cplx<R> *fhat, const R *phi_inv, int d,
                              const long *N, const long *n, hipStream_t s);
+// nb > 1: nb right-hand sides in one launch (grid y), offsets from the
+// pass's / geometry's batch strides; gb / tiles override e.geom / e.tiles
 template <typename R>
 void launch_fft_pass(cplx<R> *g, const cplx<R> *fhat_in, cplx<R> *fhat_out, const FftPass<R> &p,
-                     int sign, hipStream_t s);
+                     int sign, hipStream_t s, int nb = 1);
 template <typename R>
-void launch_interp(const cplx<R> *g, cplx<R> *f, const Engine<R> &e, hipStream_t s);
+void launch_interp(const cplx<R> *g, cplx<R> *f, const Engine<R> &e, hipStream_t s, int nb = 1,
+                   const BinGeom *gb = nullptr);
 template <typename R>
-void launch_spread(const cplx<R> *f, cplx<R> *g, const Engine<R> &e, hipStream_t s);
+void launch_spread(const cplx<R> *f, cplx<R> *g, const Engine<R> &e, hipStream_t s, int nb = 1,
+                   const BinGeom *gb = nullptr, cplx<R> *tiles = nullptr);
 template <typename R>
-void launch_gather_tiles(const Engine<R> &e, cplx<R> *g, hipStream_t s);
+void launch_gather_tiles(const Engine<R> &e, cplx<R> *g, hipStream_t s, int nb = 1,
+                         const BinGeom *gb = nullptr, cplx<R> *tiles = nullptr);
+template <typename R>
+bool batch_kernels(const Engine<R> &e);
 template <typename R>
 void launch_ndft(const R *x, const cplx<R> *in, cplx<R> *out, int d, const long *N, long M,
                  int adjoint, hipStream_t s);

@@ -211,6 +211,10 @@ template <typename R, int SIGN, int CAP, bool POW2>
 __global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
                                                   const cplx<R> *__restrict__ fin,
                                                   cplx<R> *__restrict__ fout, FftPass<R> P) {
+  // right-hand side blockIdx.y of a batched launch
+  g += (long)blockIdx.y * P.bs_g;
+  if (fin) fin += (long)blockIdx.y * P.bs_c;
+  if (fout) fout += (long)blockIdx.y * P.bs_c;
   extern __shared__ unsigned char fft_smem[];
   cplx<R> *buf = reinterpret_cast<cplx<R> *>(fft_smem);
   const int ls = lds_line_stride(P.len);
@@ -362,7 +366,7 @@ template <typename R> constexpr int fft_cap() { return sizeof(R) == 4 ? 4096 : 2
 
 template <typename R>
 void launch_fft_pass(cplx<R> *g, const cplx<R> *fhat_in, cplx<R> *fhat_out, const FftPass<R> &p,
-                     int sign, hipStream_t s) {
+                     int sign, hipStream_t s, int nb) {
   if (p.nlines <= 0) return;
   const long blocks = ceil_div(p.nlines, p.lines_per_block);
   const size_t lds = sizeof(cplx<R>) * ((size_t)p.lines_per_block * (size_t)lds_line_stride(p.len) +
@@ -372,12 +376,13 @@ void launch_fft_pass(cplx<R> *g, const cplx<R> *fhat_in, cplx<R> *fhat_out, cons
   for (int i = 0; i < p.nstages; i++) pow2 = pow2 && (p.radix[i] == 2 || p.radix[i] == 4 || p.radix[i] == 8);
   auto fn = sign < 0 ? (pow2 ? k_fft_pass<R, -1, CAP, true> : k_fft_pass<R, -1, CAP, false>)
                      : (pow2 ? k_fft_pass<R, +1, CAP, true> : k_fft_pass<R, +1, CAP, false>);
-  hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(256), lds, s, g, fhat_in, fhat_out, p);
+  hipLaunchKernelGGL(fn, dim3((unsigned)blocks, (unsigned)nb), dim3(256), lds, s, g, fhat_in,
+                     fhat_out, p);
 }
 
 template void launch_fft_pass<float>(cplx<float> *, const cplx<float> *, cplx<float> *,
-                                     const FftPass<float> &, int, hipStream_t);
+                                     const FftPass<float> &, int, hipStream_t, int);
 template void launch_fft_pass<double>(cplx<double> *, const cplx<double> *, cplx<double> *,
-                                      const FftPass<double> &, int, hipStream_t);
+                                      const FftPass<double> &, int, hipStream_t, int);
 
 }  // namespace nfft_hip

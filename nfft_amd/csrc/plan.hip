@@ -387,6 +387,8 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     // cropped to the band)
     auto make_pass = [&](int t, int in_mode, int out_mode) {
       FftPass<R> p{};
+      p.bs_g = n_total;
+      p.bs_c = N_total;
       p.len = (int)n[t];
       p.stride = gstride[t];
       p.Nband = (int)N[t];
@@ -444,6 +446,8 @@ void Engine<R>::destroy() {
   hipFree(scratch_i);
   hipFree(zone_tab);
   hipFree(aux);
+  hipFree(bgrid);
+  hipFree(btiles);
   for (auto &e : ev)
     if (e) hipEventDestroy(e);
   for (auto &set : log_ev)
@@ -526,6 +530,91 @@ void Engine<R>::adjoint(const cplx<R> *f_dev, cplx<R> *fhat_dev) {
   for (auto &ps : bwd_passes) launch_fft_pass<R>(g, nullptr, fhat_dev, ps, +1, stream);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[3], stream));
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[4], stream));
+  NFFT_HIP_CHECK(R, hipGetLastError());
+}
+
+template <typename R>
+bool Engine<R>::batchable() const {
+  return !direct_only && !timing && !deferred && M > 0 && sp_nsub > 0 && batch_kernels<R>(*this);
+}
+
+// right-hand sides per batched launch: the batch grids and tiles of one
+// launch stay below 4 GiB
+template <typename R>
+int Engine<R>::batch_chunk(int nrhs) const {
+  long tot = 1;
+  for (int t = 0; t < d; t++) tot *= geom.E[t];
+  const long per_rhs = (n_total + sp_nsub * tot) * (long)sizeof(cplx<R>);
+  const long cap = std::max(1L, (4L << 30) / std::max(1L, per_rhs));
+  return (int)std::min<long>(nrhs, cap);
+}
+
+template <typename R>
+void Engine<R>::reserve_batch(int nrhs) {
+  if (!batchable() || nrhs < 2) return;
+  const int k = batch_chunk(nrhs);
+  long tot = 1;
+  for (int t = 0; t < d; t++) tot *= geom.E[t];
+  if ((long)k * n_total > bgrid_cap) {
+    NFFT_HIP_CHECK(R, hipStreamSynchronize(stream));
+    hipFree(bgrid);
+    bgrid = nullptr;
+    NFFT_HIP_CHECK(R, hipMalloc(&bgrid, sizeof(cplx<R>) * (size_t)k * n_total));
+    bgrid_cap = (long)k * n_total;
+  }
+  if ((long)k * sp_nsub * tot > btiles_cap) {
+    NFFT_HIP_CHECK(R, hipStreamSynchronize(stream));
+    hipFree(btiles);
+    btiles = nullptr;
+    NFFT_HIP_CHECK(R, hipMalloc(&btiles, sizeof(cplx<R>) * (size_t)k * sp_nsub * tot));
+    btiles_cap = (long)k * sp_nsub * tot;
+  }
+}
+
+// nrhs transforms with the plan's nodes: right-hand side r of fhat_dev at
+// r N_total, of f_dev at r M; each kernel of the path runs once for a chunk
+// of right-hand sides (grid y), on the batch grids
+template <typename R>
+void Engine<R>::trafo_batch(const cplx<R> *fhat_dev, cplx<R> *f_dev, int nrhs) {
+  if (nrhs < 2 || !batchable()) {
+    for (int r = 0; r < nrhs; r++) trafo(fhat_dev + (long)r * N_total, f_dev + (long)r * M);
+    return;
+  }
+  reserve_batch(nrhs);
+  const int k = batch_chunk(nrhs);
+  BinGeom gb = geom;
+  gb.bs_g = n_total;
+  gb.bs_f = M;
+  for (int r0 = 0; r0 < nrhs; r0 += k) {
+    const int nb = std::min(k, nrhs - r0);
+    for (auto &ps : fwd_passes)
+      launch_fft_pass<R>(bgrid, fhat_dev + (long)r0 * N_total, nullptr, ps, -1, stream, nb);
+    launch_interp<R>(bgrid, f_dev + (long)r0 * M, *this, stream, nb, &gb);
+  }
+  NFFT_HIP_CHECK(R, hipGetLastError());
+}
+
+template <typename R>
+void Engine<R>::adjoint_batch(const cplx<R> *f_dev, cplx<R> *fhat_dev, int nrhs) {
+  if (nrhs < 2 || !batchable()) {
+    for (int r = 0; r < nrhs; r++) adjoint(f_dev + (long)r * M, fhat_dev + (long)r * N_total);
+    return;
+  }
+  reserve_batch(nrhs);
+  const int k = batch_chunk(nrhs);
+  long tot = 1;
+  for (int t = 0; t < d; t++) tot *= geom.E[t];
+  BinGeom gb = geom;
+  gb.bs_g = n_total;
+  gb.bs_f = M;
+  gb.bs_t = sp_nsub * tot;
+  for (int r0 = 0; r0 < nrhs; r0 += k) {
+    const int nb = std::min(k, nrhs - r0);
+    launch_spread<R>(f_dev + (long)r0 * M, bgrid, *this, stream, nb, &gb, btiles);
+    launch_gather_tiles<R>(*this, bgrid, stream, nb, &gb, btiles);
+    for (auto &ps : bwd_passes)
+      launch_fft_pass<R>(bgrid, nullptr, fhat_dev + (long)r0 * N_total, ps, +1, stream, nb);
+  }
   NFFT_HIP_CHECK(R, hipGetLastError());
 }
 
@@ -1015,14 +1104,30 @@ struct PlanApi {
   }
 
   // nrhs transforms sharing the plan's nodes: vectors r of f_hat_dev / f_dev
-  // at offsets r N_total / r M_total, enqueued back to back
+  // at offsets r N_total / r M_total, in batched launches (one launch per
+  // kernel for a chunk of right-hand sides) where the plan's kernels allow,
+  // else enqueued back to back
   static void trafo_batch_device(PLAN *p, const void *fh, void *f, int nrhs) {
-    for (int r = 0; r < nrhs; r++)
-      trafo_device(p, (const cplx<R> *)fh + (long)r * p->N_total, (cplx<R> *)f + (long)r * p->M_total);
+    Engine<R> *e = eng(p);
+    if (nrhs < 2 || e->direct_only || e->timing) {
+      for (int r = 0; r < nrhs; r++)
+        trafo_device(p, (const cplx<R> *)fh + (long)r * p->N_total, (cplx<R> *)f + (long)r * p->M_total);
+      return;
+    }
+    clear_launch_error();
+    ensure_nodes(p, e);
+    e->trafo_batch((const cplx<R> *)fh, (cplx<R> *)f, nrhs);
   }
   static void adjoint_batch_device(PLAN *p, const void *f, void *fh, int nrhs) {
-    for (int r = 0; r < nrhs; r++)
-      adjoint_device(p, (const cplx<R> *)f + (long)r * p->M_total, (cplx<R> *)fh + (long)r * p->N_total);
+    Engine<R> *e = eng(p);
+    if (nrhs < 2 || e->direct_only || e->timing) {
+      for (int r = 0; r < nrhs; r++)
+        adjoint_device(p, (const cplx<R> *)f + (long)r * p->M_total, (cplx<R> *)fh + (long)r * p->N_total);
+      return;
+    }
+    clear_launch_error();
+    ensure_nodes(p, e);
+    e->adjoint_batch((const cplx<R> *)f, (cplx<R> *)fh, nrhs);
   }
 
   // One hipGraph for the batched transforms of `count` plans: captured on
@@ -1042,6 +1147,7 @@ struct PlanApi {
         if (!e->nodes_ready) precompute(plans[i]);
       } else {
         ensure_nodes(plans[i], e);
+        e->reserve_batch(nrhs);  // batch buffers exist before the capture
       }
     }
     hipStream_t s0 = eng(plans[0])->stream;
@@ -1061,17 +1167,22 @@ struct PlanApi {
       Engine<R> *e = eng(p);
       hipStream_t si = e->stream;
       if (si != s0) NFFT_HIP_CHECK(R, hipStreamWaitEvent(si, fork, 0));
-      for (int r = 0; r < nrhs; r++) {
-        if (!adjoint) {
-          const cplx<R> *fh = (const cplx<R> *)in[i] + (long)r * p->N_total;
-          cplx<R> *f = (cplx<R> *)out[i] + (long)r * p->M_total;
-          if (e->direct_only) e->trafo_direct(fh, f);
-          else e->trafo(fh, f);
-        } else {
-          const cplx<R> *f = (const cplx<R> *)in[i] + (long)r * p->M_total;
-          cplx<R> *fh = (cplx<R> *)out[i] + (long)r * p->N_total;
-          if (e->direct_only) e->adjoint_direct(f, fh);
-          else e->adjoint(f, fh);
+      if (!e->direct_only && nrhs > 1) {
+        if (!adjoint) e->trafo_batch((const cplx<R> *)in[i], (cplx<R> *)out[i], nrhs);
+        else e->adjoint_batch((const cplx<R> *)in[i], (cplx<R> *)out[i], nrhs);
+      } else {
+        for (int r = 0; r < nrhs; r++) {
+          if (!adjoint) {
+            const cplx<R> *fh = (const cplx<R> *)in[i] + (long)r * p->N_total;
+            cplx<R> *f = (cplx<R> *)out[i] + (long)r * p->M_total;
+            if (e->direct_only) e->trafo_direct(fh, f);
+            else e->trafo(fh, f);
+          } else {
+            const cplx<R> *f = (const cplx<R> *)in[i] + (long)r * p->M_total;
+            cplx<R> *fh = (cplx<R> *)out[i] + (long)r * p->N_total;
+            if (e->direct_only) e->adjoint_direct(f, fh);
+            else e->adjoint(f, fh);
+          }
         }
       }
       if (si != s0) {

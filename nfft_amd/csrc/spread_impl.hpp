@@ -288,6 +288,9 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
                                                      const R *__restrict__ frac,
                                                      const int4 *__restrict__ subs, BinGeom gg,
                                                      WinParams<R> wp, cplx<R> *__restrict__ tiles) {
+  // right-hand side blockIdx.y of a batched launch
+  g += (long)blockIdx.y * gg.bs_g;
+  fout += (long)blockIdx.y * gg.bs_f;
   extern __shared__ unsigned char b_smem[];
   cplx<R> *tile = reinterpret_cast<cplx<R> *>(b_smem);
   const int4 sp = subs[blockIdx.x];
@@ -1169,6 +1172,9 @@ __global__ __launch_bounds__(256) void k_gather_zones(const cplx<R> *__restrict_
                                                       const int *__restrict__ sp_count,
                                                       cplx<R> *__restrict__ g, BinGeom gg,
                                                       const int *__restrict__ order) {
+  // right-hand side blockIdx.y of a batched launch
+  tiles += (long)blockIdx.y * gg.bs_t;
+  g += (long)blockIdx.y * gg.bs_g;
   constexpr int NZ = D == 1 ? 3 : (D == 2 ? 9 : 27);
   constexpr int BATCH = CPT >= 16 ? 1 : 16 / CPT;
   __shared__ long s_base[NZ];   // offset of the zone inside a tile
@@ -1296,9 +1302,25 @@ __global__ __launch_bounds__(256) void k_gather_zones(const cplx<R> *__restrict_
 }
 
 template <typename R>
-void launch_gather_tiles(const Engine<R> &e, cplx<R> *g, hipStream_t s) {
+void launch_gather_tiles(const Engine<R> &e, cplx<R> *g, hipStream_t s, int nb, const BinGeom *gb,
+                         cplx<R> *tiles) {
   long nbins = 1;
   for (int t = 0; t < e.geom.d; t++) nbins *= e.geom.nb[t];
+  if (nb > 1) {
+    // batched: the zone reduction reads the right-hand side from blockIdx.y
+    using F = void (*)(const cplx<R> *, const int *, const int *, cplx<R> *, BinGeom, const int *);
+    long core = 1;
+    for (int t = 0; t < e.geom.d; t++) core *= e.geom.bs[t];
+    F fz = e.geom.d == 3 ? (core <= 512 ? k_gather_zones<R, 3, 2>
+                                        : (core <= 1024 ? k_gather_zones<R, 3, 4>
+                                                        : k_gather_zones<R, 3, 16>))
+                         : (core <= 256 ? k_gather_zones<R, 2, 1>
+                                        : (core <= 1024 ? k_gather_zones<R, 2, 4>
+                                                        : k_gather_zones<R, 2, 16>));
+    hipLaunchKernelGGL(fz, dim3((unsigned)nbins, (unsigned)nb), dim3(256), 0, s, tiles, e.sp_first,
+                       e.sp_count, g, *gb, e.bin_order);
+    return;
+  }
   if (e.geom.zoned) {
     long core = 1;
     for (int t = 0; t < e.geom.d; t++) core *= e.geom.bs[t];
@@ -1484,26 +1506,48 @@ const BKernelTable<R> &bkernels() {
 }
 
 template <typename R>
-void launch_interp(const cplx<R> *g, cplx<R> *f, const Engine<R> &e, hipStream_t s) {
+void launch_interp(const cplx<R> *g, cplx<R> *f, const Engine<R> &e, hipStream_t s, int nb,
+                   const BinGeom *gb) {
   if (e.nsub == 0) return;
+  const BinGeom &gg = gb ? *gb : e.geom;
   size_t lds;
   int threads;
-  auto fn = pick_bkernel<R>(bkernels<R>(), e.geom, false, e.interp_variant, lds, threads,
+  auto fn = pick_bkernel<R>(bkernels<R>(), gg, false, e.interp_variant, lds, threads,
                             e.win.poly_shared != 0 && e.sym_window);
-  hipLaunchKernelGGL(fn, dim3((unsigned)e.nsub), dim3(threads), lds, s, g, f, e.perm, e.cell, e.frac,
-                     e.subs, e.geom, e.win, (cplx<R> *)nullptr);
+  hipLaunchKernelGGL(fn, dim3((unsigned)e.nsub, (unsigned)nb), dim3(threads), lds, s, g, f, e.perm,
+                     e.cell, e.frac, e.subs, gg, e.win, (cplx<R> *)nullptr);
 }
 
 template <typename R>
-void launch_spread(const cplx<R> *f, cplx<R> *g, const Engine<R> &e, hipStream_t s) {
+void launch_spread(const cplx<R> *f, cplx<R> *g, const Engine<R> &e, hipStream_t s, int nb,
+                   const BinGeom *gb, cplx<R> *tiles) {
   if (e.sp_nsub == 0) return;
+  const BinGeom &gg = gb ? *gb : e.geom;
   size_t lds;
   int threads;
-  auto fn = pick_bkernel<R>(bkernels<R>(), e.geom, true, e.spread_variant, lds, threads);
+  auto fn = pick_bkernel<R>(bkernels<R>(), gg, true, e.spread_variant, lds, threads);
   // the scratch reduction needs an LDS tile; the global variant adds directly
-  cplx<R> *tiles = (lds > 0 && e.tile_scratch) ? e.tiles : nullptr;
-  hipLaunchKernelGGL(fn, dim3((unsigned)e.sp_nsub), dim3(threads), lds, s, f, g, e.perm, e.cell,
-                     e.frac, e.sp_subs, e.geom, e.win, tiles);
+  if (!tiles) tiles = (lds > 0 && e.tile_scratch) ? e.tiles : nullptr;
+  hipLaunchKernelGGL(fn, dim3((unsigned)e.sp_nsub, (unsigned)nb), dim3(threads), lds, s, f, g,
+                     e.perm, e.cell, e.frac, e.sp_subs, gg, e.win, tiles);
+}
+
+// Whether the batched launches above cover this plan: the kernels that read
+// their right-hand side from blockIdx.y are the FP32 defaults (lane-per-node
+// interpolation, the run-merging spreading kernels with scratch tiles) and
+// the zone reduction.
+template <typename R>
+bool batch_kernels(const Engine<R> &e) {
+  if constexpr (!std::is_same_v<R, float>) {
+    return false;
+  } else {
+    const BinGeom &gg = e.geom;
+    if (!(e.tile_scratch && gg.zoned && gg.m >= 1 && gg.m <= kMaxStaticM)) return false;
+    if (tile_bytes<R>(gg) > kTileBudget || e.interp_variant != 0) return false;
+    if (gg.d == 3) return e.spread_variant == 10 && bkernels<R>().spread_runs_2sbc[gg.m];
+    if (gg.d == 2) return e.spread_variant == 14 && bkernels<R>().spread_runs2d[gg.m];
+    return false;
+  }
 }
 
 }  // namespace nfft_hip

@@ -37,6 +37,10 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const float2 *__restrict__
   constexpr int NP = 2 * W <= 32 ? 2 : 1;  // nodes per window pass
   constexpr int NPASS = G / NP;
   static_assert(G % NP == 0 && 64 % G == 0, "groups tile the 64-node chunk");
+  // right-hand side blockIdx.y of a batched launch
+  fin += (long)blockIdx.y * gg.bs_f;
+  g += (long)blockIdx.y * gg.bs_g;
+  if (tiles) tiles += (long)blockIdx.y * gg.bs_t;
   extern __shared__ unsigned char b_smem[];
   float2 *tile = reinterpret_cast<float2 *>(b_smem);
   const int4 sp = subs[blockIdx.x];

@@ -41,12 +41,17 @@ def rand_dev(n, seed, precision):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["single", "double"])
-def test_batch_matches_single_calls(precision):
-    N = [32, 32]
-    x = W.golden_radial(50, 64)
+@pytest.mark.parametrize("N,gen", [([32, 32], lambda: W.golden_radial(50, 64)),
+                                   ([16, 16, 16], lambda: W.kooshball(64, 64, seed=3))],
+                         ids=["2d", "3d"])
+def test_batch_matches_single_calls(precision, N, gen):
+    """FP32 plans with the default kernels run the batch as one launch per
+    kernel (grid y = right-hand side); the others loop.  Either way each
+    right-hand side equals its single call."""
+    x = gen()
     p = plan(N, x, precision)
     try:
-        nrhs, NN, M = 5, 32 * 32, p.M
+        nrhs, NN, M = 5, int(np.prod(N)), p.M
         fh = rand_dev(nrhs * NN, 1, precision)
         f = torch.empty(nrhs * M, dtype=cdt(precision), device="cuda")
         p.trafo_batch_device(fh.data_ptr(), f.data_ptr(), nrhs)
