@@ -655,6 +655,28 @@ void Engine<R>::autotune() {
     }
     spread_variant = pick;
   }
+  // spreading work-item size (nodes per item): half and twice the default,
+  // the work lists rebuilt for each (fewer, longer items write fewer tiles
+  // to reduce; more, shorter ones balance dense regions better)
+  if (!std::getenv("NFFT_HIP_SPREAD_ITEM") && d >= 2 && tile_scratch && sp_nsub > 0) {
+    const int base = spread_max_per_sub;
+    float best = time_of([&]() { spread_to_grid(f); });
+    int pick = base;
+    for (int c : {base / 2, base * 2}) {
+      if (c < 256 || c > 8192) continue;
+      spread_max_per_sub = c;
+      build_nodes();
+      const float ms = time_of([&]() { spread_to_grid(f); });
+      if (ms < best) {
+        best = ms;
+        pick = c;
+      }
+    }
+    if (spread_max_per_sub != pick) {
+      spread_max_per_sub = pick;
+      build_nodes();
+    }
+  }
   if (!std::getenv("NFFT_HIP_INTERP") && d >= 2) {
     float best = 1e30f;
     int pick = interp_variant;
