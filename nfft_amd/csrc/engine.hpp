@@ -42,6 +42,11 @@ struct BinGeom {
   // batched launches (blockIdx.y = right-hand side): per right-hand side
   // offsets of the grid, the node vector and the scratch tiles; 0 otherwise
   long bs_g, bs_f, bs_t;
+  // interpolation slots (see nodes.hip): slots[k] = (node, second node of
+  // the same cell or -1); islots[item] = (first slot, slot count); null when
+  // the plan's interpolation kernel does not use them
+  const int2 *slots;
+  const int2 *islots;
 };
 
 // Kaiser-Bessel window of one dimension, scaled by 1/phi(0) so that the B
@@ -154,6 +159,14 @@ template <typename R> struct Engine {
   bool tile_scratch = false;
   int *scratch_i = nullptr;  // binning scratch
   int *zone_tab = nullptr;   // geom.zmap and geom.zdesc
+  int2 *slots = nullptr, *islots = nullptr;  // geom.slots, geom.islots
+  long slots_cap = 0, islots_cap = 0, nslots = 0;
+  // FP32 3-D interpolation reads the taps of two same-cell nodes once
+  // (NFFT_HIP_PAIRS=0: off).  In 2-D the extra window work outweighs the
+  // saved LDS reads (cfg2-like plans ran 64% slower in B), so 2-D keeps the
+  // single-node kernel.
+  int pair_interp = 1;
+  bool use_slots() const { return sizeof(R) == 4 && d == 3 && pair_interp; }
   // device scratch of the density-weight helpers (features, Pipe-Menon),
   // grown on demand and kept with the plan so repeated calls allocate nothing
   char *aux = nullptr;

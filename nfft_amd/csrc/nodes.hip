@@ -247,6 +247,58 @@ __global__ void k_sub_order(const int4 *__restrict__ in, const int *__restrict__
   out[atomicAdd(&bucket_next[bucket_of[i]], 1)] = in[i];
 }
 
+// Interpolation slots: a slot is one node, or two consecutive nodes of one
+// cell run (same cell, same work item) whose grid taps the kernel reads once.
+// Runs start where the cell changes or a work item starts; slots start at
+// the even positions of each run.
+__global__ void k_mark_item_starts(const int4 *__restrict__ subs, long nsub, int *__restrict__ b) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nsub) b[subs[i].y] = 1;
+}
+
+__global__ void k_run_flags(const int *__restrict__ cell, int d, long M, int *__restrict__ b) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  bool start = j == 0 || b[j] != 0;
+  for (int t = 0; t < d && !start; t++) start = cell[t * M + j] != cell[t * M + j - 1];
+  b[j] = start ? 1 : 0;
+}
+
+// run_id[j] = (exclusive sum of b)[j] + b[j] - 1 in place; run_start[id] = j
+__global__ void k_run_starts(const int *__restrict__ b, int *__restrict__ run_id, long M,
+                             int *__restrict__ run_start) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  const int id = run_id[j] + b[j] - 1;
+  run_id[j] = id;
+  if (b[j]) run_start[id] = (int)j;
+}
+
+__global__ void k_slot_flags(const int *__restrict__ run_id, const int *__restrict__ run_start,
+                             long M, int *__restrict__ sflag) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  sflag[j] = ((j - run_start[run_id[j]]) & 1) == 0 ? 1 : 0;
+}
+
+__global__ void k_fill_slots(const int *__restrict__ b, const int *__restrict__ sflag,
+                             const int *__restrict__ sidx, long M, int2 *__restrict__ slots) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M || !sflag[j]) return;
+  const bool pair = j + 1 < M && b[j + 1] == 0;
+  slots[sidx[j]] = make_int2((int)j, pair ? (int)(j + 1) : -1);
+}
+
+__global__ void k_item_slots(const int4 *__restrict__ subs, long nsub, const int *__restrict__ sidx,
+                             long M, int nslots, int2 *__restrict__ islots) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nsub) return;
+  const long first = subs[i].y, end = first + subs[i].z;
+  const int s0 = sidx[first];
+  const int s1 = end < M ? sidx[end] : nslots;
+  islots[i] = make_int2(s0, s1 - s0);
+}
+
 __global__ void k_fill_zero_int(int *__restrict__ p, long count) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = 0;
@@ -278,7 +330,8 @@ void bin_nodes(Engine<R> &e) {
   // frac_tmp (d M reals), sums (256 ints: scan totals, ordering buckets)
   auto al = [](long bytes) { return (bytes + 255) / 256 * 256; };
   const long b_keys = al(nkeys * (long)sizeof(int));
-  const long b_tiles = al(ntiles * (long)sizeof(int));
+  // the tile sums also serve scans of M-long node arrays
+  const long b_tiles = al(std::max(ntiles, ceil_div(M + 1, kScanTile)) * (long)sizeof(int));
   const long b_bins = al(nbins * (long)sizeof(int));
   const long b_M = al(M * (long)sizeof(int));
   const long b_cell = al((long)gg.d * M * (long)sizeof(int));
@@ -362,6 +415,54 @@ void bin_nodes(Engine<R> &e) {
     }
   };
   make_list(e.max_per_sub, sub_start, nsub_of, &e.subs, &e.subs_cap, &e.nsub);
+  // interpolation slots (pairs of same-cell nodes) for the kernels that use them
+  e.geom.slots = e.geom.islots = nullptr;
+  e.nslots = 0;
+  if (e.use_slots() && M > 0 && e.nsub > 0) {
+    auto long_scan = [&](const int *in, int *out, long len, int *total) {
+      const long nt = ceil_div(len, kScanTile);
+      hipLaunchKernelGGL(k_scan_tile_sums, dim3((unsigned)nt), dim3(256), 0, s, in, len, tile_sum);
+      hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, tile_sum, tile_off, nt,
+                         total);
+      hipLaunchKernelGGL(k_scan_tile_apply, dim3((unsigned)nt), dim3(256), 0, s, in, out, len,
+                         tile_off);
+    };
+    // node-length temporaries (free after placement): run-start flags,
+    // run ids, run starts, slot flags, slot indices
+    int *b = key_of;
+    int *run_id = rank_of;
+    int *run_start = cell_tmp;
+    int *sflag = cell_tmp + M;
+    int *sidx = reinterpret_cast<int *>(frac_tmp);
+    const unsigned gm = (unsigned)ceil_div(M, 256);
+    launch_fill_zero_int(b, M, s);
+    hipLaunchKernelGGL(k_mark_item_starts, dim3((unsigned)ceil_div(e.nsub, 256)), dim3(256), 0, s,
+                       e.subs, e.nsub, b);
+    hipLaunchKernelGGL(k_run_flags, dim3(gm), dim3(256), 0, s, e.cell, gg.d, M, b);
+    long_scan(b, run_id, M, sums + 100);
+    hipLaunchKernelGGL(k_run_starts, dim3(gm), dim3(256), 0, s, b, run_id, M, run_start);
+    hipLaunchKernelGGL(k_slot_flags, dim3(gm), dim3(256), 0, s, run_id, run_start, M, sflag);
+    long_scan(sflag, sidx, M, sums + 101);
+    int nslots = 0;
+    NFFT_HIP_CHECK(R, hipMemcpyAsync(&nslots, sums + 101, sizeof(int), hipMemcpyDeviceToHost, s));
+    NFFT_HIP_CHECK(R, hipStreamSynchronize(s));
+    if (nslots > e.slots_cap) {
+      if (e.slots) NFFT_HIP_CHECK(R, hipFree(e.slots));
+      NFFT_HIP_CHECK(R, hipMalloc(&e.slots, sizeof(int2) * (size_t)nslots));
+      e.slots_cap = nslots;
+    }
+    if (e.nsub > e.islots_cap) {
+      if (e.islots) NFFT_HIP_CHECK(R, hipFree(e.islots));
+      NFFT_HIP_CHECK(R, hipMalloc(&e.islots, sizeof(int2) * (size_t)e.nsub));
+      e.islots_cap = e.nsub;
+    }
+    hipLaunchKernelGGL(k_fill_slots, dim3(gm), dim3(256), 0, s, b, sflag, sidx, M, e.slots);
+    hipLaunchKernelGGL(k_item_slots, dim3((unsigned)ceil_div(e.nsub, 256)), dim3(256), 0, s,
+                       e.subs, e.nsub, sidx, M, nslots, e.islots);
+    e.nslots = nslots;
+    e.geom.slots = e.slots;
+    e.geom.islots = e.islots;
+  }
   if (nbins > e.bin_arrays_cap) {
     if (e.sp_first) NFFT_HIP_CHECK(R, hipFree(e.sp_first));
     NFFT_HIP_CHECK(R, hipMalloc(&e.sp_first, 3 * nbins * sizeof(int)));

@@ -339,6 +339,7 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   };
   interp_variant = variant("NFFT_HIP_INTERP", 0);
   if (const char *v = std::getenv("NFFT_HIP_SYMWIN")) sym_window = std::atoi(v) != 0;
+  if (const char *v = std::getenv("NFFT_HIP_PAIRS")) pair_interp = std::atoi(v) != 0;
   // float: plain FP32 read-modify-write by wavefronts that own disjoint
   // cells, same-cell runs summed in registers (3-D: two wavefronts per tile
   // split by row parity; 2-D: one wavefront per tile; 1-D: private tiles);
@@ -448,6 +449,8 @@ void Engine<R>::destroy() {
   hipFree(aux);
   hipFree(bgrid);
   hipFree(btiles);
+  hipFree(slots);
+  hipFree(islots);
   for (auto &e : ev)
     if (e) hipEventDestroy(e);
   for (auto &set : log_ev)

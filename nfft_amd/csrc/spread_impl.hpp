@@ -280,7 +280,10 @@ __device__ __forceinline__ void load_region_3d(const cplx<R> *__restrict__ g, cp
   }
 }
 
-template <typename R, int D, int W, bool SYM = false>
+// PAIR (D = 2, 3): threads take the work item's slots (gg.islots / gg.slots)
+// instead of single nodes; a slot of two nodes of one cell reads each grid
+// tap once from LDS for both, halving the LDS bytes per node of dense runs.
+template <typename R, int D, int W, bool SYM = false, bool PAIR = false>
 __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__ g,
                                                      cplx<R> *__restrict__ fout,
                                                      const int *__restrict__ perm,
@@ -308,6 +311,74 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
   const int E1 = D >= 2 ? gg.E[1] : 1;
   const int E2 = D == 3 ? gg.E[2] : 1;
   __syncthreads();
+  if constexpr (PAIR && (D == 2 || D == 3)) {
+    const int2 is = gg.islots[blockIdx.x];
+    for (int k = threadIdx.x; k < is.y; k += blockDim.x) {
+      const int2 pr = gg.slots[is.x + k];
+      const long pa = pr.x;
+      const bool two = pr.y >= 0;
+      const long pb = two ? pr.y : pr.x;
+      int c[D], cb[D];
+      R pa_[D][W], pb_[D][W];
+      if constexpr (SYM) {
+        node_window_sym<D, W>(cell, frac, pa, gg.M, wp, c, pa_);
+        node_window_sym<D, W>(cell, frac, pb, gg.M, wp, cb, pb_);
+      } else {
+        node_window<R, D, W>(cell, frac, pa, gg.M, wp, c, pa_);
+        node_window<R, D, W>(cell, frac, pb, gg.M, wp, cb, pb_);
+      }
+      R ar = 0, ai = 0, br = 0, bi = 0;
+      if constexpr (D == 2) {
+        const cplx<R> *base = tile + (c[0] - lo[0]) * E1 + (c[1] - lo[1]);
+#pragma unroll
+        for (int i0 = 0; i0 < W; i0++) {
+          const cplx<R> *row = base + i0 * E1;
+          R sar = 0, sai = 0, sbr = 0, sbi = 0;
+#pragma unroll
+          for (int w = 0; w < W; w++) {
+            const cplx<R> v = row[w];
+            sar += pa_[1][w] * v.x;
+            sai += pa_[1][w] * v.y;
+            sbr += pb_[1][w] * v.x;
+            sbi += pb_[1][w] * v.y;
+          }
+          ar += pa_[0][i0] * sar;
+          ai += pa_[0][i0] * sai;
+          br += pb_[0][i0] * sbr;
+          bi += pb_[0][i0] * sbi;
+        }
+      } else {
+        const cplx<R> *base = tile + ((c[0] - lo[0]) * E1 + (c[1] - lo[1])) * E2 + (c[2] - lo[2]);
+        for (int i0 = 0; i0 < W; i0++) {
+          R qar = 0, qai = 0, qbr = 0, qbi = 0;
+#pragma unroll
+          for (int i1 = 0; i1 < W; i1++) {
+            const cplx<R> *row = base + (i0 * E1 + i1) * E2;
+            R sar = 0, sai = 0, sbr = 0, sbi = 0;
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+              const cplx<R> v = row[w];
+              sar += pa_[2][w] * v.x;
+              sai += pa_[2][w] * v.y;
+              sbr += pb_[2][w] * v.x;
+              sbi += pb_[2][w] * v.y;
+            }
+            qar += pa_[1][i1] * sar;
+            qai += pa_[1][i1] * sai;
+            qbr += pb_[1][i1] * sbr;
+            qbi += pb_[1][i1] * sbi;
+          }
+          ar += pa_[0][i0] * qar;
+          ai += pa_[0][i0] * qai;
+          br += pb_[0][i0] * qbr;
+          bi += pb_[0][i0] * qbi;
+        }
+      }
+      fout[perm[pa]] = cplx<R>{ar, ai};
+      if (two) fout[perm[pb]] = cplx<R>{br, bi};
+    }
+    return;
+  }
   for (int i = threadIdx.x; i < sp.z; i += blockDim.x) {
     const long p = (long)sp.y + i;
     int c[D];
@@ -1353,6 +1424,8 @@ template <typename R> struct BKernelTable {
                       const int4 *, BinGeom, WinParams<R>, cplx<R> *);
   fn interp_static[kMaxDim][kMaxStaticM + 1];
   fn interp_sym[kMaxDim][kMaxStaticM + 1];   // float only, shared tap table
+  fn interp_pair[kMaxDim][kMaxStaticM + 1];  // float d = 2, 3: node pairs of one cell
+  fn interp_pair_sym[kMaxDim][kMaxStaticM + 1];
   fn spread_static[kMaxDim][kMaxStaticM + 1];
   fn interp_wave[kMaxDim][kMaxStaticM + 1];  // d = 2, 3 only
   fn spread_wave[kMaxDim][kMaxStaticM + 1];
@@ -1376,6 +1449,10 @@ template <typename R, int D, int M_>
 void bk_fill_one(BKernelTable<R> &k) {
   k.interp_static[D - 1][M_] = k_interp_tile<R, D, 2 * M_ + 2>;
   if constexpr (std::is_same_v<R, float>) k.interp_sym[D - 1][M_] = k_interp_tile<R, D, 2 * M_ + 2, true>;
+  if constexpr (std::is_same_v<R, float> && (D == 2 || D == 3)) {
+    k.interp_pair[D - 1][M_] = k_interp_tile<R, D, 2 * M_ + 2, false, true>;
+    k.interp_pair_sym[D - 1][M_] = k_interp_tile<R, D, 2 * M_ + 2, true, true>;
+  }
   k.spread_static[D - 1][M_] = k_spread_tile<R, D, 2 * M_ + 2>;
   if constexpr (D >= 2 && D * (2 * M_ + 2) <= 64) {
     k.interp_wave[D - 1][M_] = k_bwave_tile<R, D, 2 * M_ + 2, false>;
@@ -1493,6 +1570,8 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
       auto fn = spread ? k.spread_wave[t][gg.m] : k.interp_wave[t][gg.m];
       if (fn) return fn;
     }
+    if (!spread && variant == 0 && gg.slots && k.interp_pair[t][gg.m])
+      return shared_poly ? k.interp_pair_sym[t][gg.m] : k.interp_pair[t][gg.m];
     if (!spread && shared_poly && k.interp_sym[t][gg.m]) return k.interp_sym[t][gg.m];
     return spread ? k.spread_static[t][gg.m] : k.interp_static[t][gg.m];
   }
