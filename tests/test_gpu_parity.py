@@ -276,3 +276,17 @@ def test_four_dimensional(precision, m, N, n):
     print("4-D m=%d %s trafo l2 %.2e adjoint l2 %.2e" % (m, precision, l2t, l2a))
     tol = 1e-12 if precision == "double" else 1e-5
     assert l2t < tol and l2a < tol
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pairs", ["0", "1"])
+def test_interp_pairs_on_and_off(pairs, monkeypatch):
+    """3-D FP32 interpolation with and without same-cell node pairs (dense
+    kooshball centre: most nodes share their cell) against the oracle."""
+    monkeypatch.setenv("NFFT_HIP_PAIRS", pairs)
+    N = [16, 16, 16]
+    x = W.kooshball(96, 64, seed=5)
+    fh = W.complex_normal(int(np.prod(N)), seed=81)
+    f = W.complex_normal(x.shape[0], seed=82)
+    (fg, fr, _), (hg, hr, _) = run_pair(N, x, fh, f, 4, "single")
+    assert rel_errs(fg, fr)[0] < 1e-5 and rel_errs(hg, hr)[0] < 1e-5
