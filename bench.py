@@ -33,6 +33,8 @@ Also reported:
                 single-threaded on the host
   batched       32 coil right-hand sides on one 2-D plan and 64 plans of
                 uneven size, eager and as hipGraph replays
+  other_configs the 1-D FP64 and 2-D (512^2, FP32 / FP64, m 4 / 6) single-plan
+                workloads: ms per trafo+adjoint (with --no-batch skipped too)
   cgnr          the metric's second part on held-out 2-D trajectories
                 (N = 256^2, sigma 2, m = 4, FP32; golden-angle radial and
                 variable-density spiral sets; phantom of random ellipses):
@@ -213,6 +215,61 @@ def cgnr_leg(args):
         out[crit] = {"total_s": t,
                      "speedup_learned_vs_plain": (t["plain"] / t["learned"]
                                                   if t["plain"] and t["learned"] else None)}
+    return out
+
+
+def configs_leg(args):
+    """The other single-plan workloads, device-resident, precompute excluded:
+    1-D FP64 (N=1024, sigma 2, m=6, M=4096 uniform nodes) and 2-D N=512^2
+    (sigma 2, M=2^20: half uniform, half Gaussian sd 0.05 at the origin) in
+    FP32 and FP64 with m = 4 and 6.  Per case: ms per trafo+adjoint, nodes/s
+    and the algorithmic HBM bytes of both directions over that time."""
+    import torch
+    import nfft_amd as nf
+    from nfft_amd import workloads as W
+    flags = nf.PRE_PHI_HUT | nf.PRE_PSI | nf.MALLOC_X | nf.MALLOC_F_HAT | nf.MALLOC_F | nf.FFTW_INIT
+    r = np.random.default_rng(2)
+    M2 = 1 << 20
+    x2 = np.concatenate([r.random((M2 // 2, 2)) - 0.5, 0.05 * r.standard_normal((M2 // 2, 2))])
+    x2 = x2 - np.round(x2)
+    x2[x2 >= 0.5] -= 1.0
+    cases = [("1d_fp64_m6", [1024], [2048], 6, W.uniform(4096, 1, seed=11), "double")]
+    for prec in ("single", "double"):
+        for m in (4, 6):
+            cases.append(("2d_%s_m%d" % ("fp32" if prec == "single" else "fp64", m), [512, 512],
+                          [1024, 1024], m, x2, prec))
+    out = {}
+    for name, N, n, m, x, prec in cases:
+        M = x.shape[0]
+        p = nf.Plan(N, M, n=n, m=m, flags=flags, precision=prec)
+        p.x[:] = x
+        p.set_stream(torch.cuda.current_stream().cuda_stream)
+        p.precompute()
+        ct = torch.complex64 if prec == "single" else torch.complex128
+        NN = int(np.prod(N))
+        fh = torch.randn(NN, dtype=ct, device="cuda")
+        f = torch.randn(M, dtype=ct, device="cuda")
+        fo = torch.empty(M, dtype=ct, device="cuda")
+        ho = torch.empty(NN, dtype=ct, device="cuda")
+
+        def step():
+            p.trafo_device(fh.data_ptr(), fo.data_ptr())
+            p.adjoint_device(f.data_ptr(), ho.data_ptr())
+        for _ in range(3):
+            step()
+        torch.cuda.synchronize()
+        K = 20
+        t0 = time.perf_counter()
+        for _ in range(K):
+            step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / K
+        c = 8 if prec == "single" else 16
+        ab = algorithmic_bytes(N, n, M, len(N), c, c // 2)
+        out[name] = {"ms": dt * 1e3, "nodes_per_s": M / dt, "M": M, "N": N, "m": m,
+                     "algorithmic_GBs": (ab["trafo"] + ab["adjoint"]) / dt / 1e9}
+        log("config %s: %.3f ms per trafo+adjoint" % (name, dt * 1e3))
+        p.finalize()
     return out
 
 
@@ -485,6 +542,7 @@ def main():
     p.finalize()
     if not args.no_batch:
         out["batched"] = batch_leg(args)
+        out["other_configs"] = configs_leg(args)
     if not args.no_cgnr:
         out["cgnr"] = cgnr_leg(args)
     print(json.dumps(out))
