@@ -645,6 +645,7 @@ void Engine<R>::autotune() {
     std::vector<int> cand;
     if (sizeof(R) == 4 && d == 3) cand = {10, 11, 6, 8, 9, 7, 4, 5, 3};
     else if (sizeof(R) == 4 && d == 2) cand = {14, 1, 0, 2};
+    else if (sizeof(R) == 8 && d == 2) cand = {14, 1, 0};
     else if (sizeof(R) == 8 && d >= 2) cand = {1, 0};
     float best = 1e30f;
     int pick = spread_variant;

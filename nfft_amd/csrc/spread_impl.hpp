@@ -1437,7 +1437,7 @@ template <typename R> struct BKernelTable {
   fn spread_runs_2st[kMaxStaticM + 1];        // same, row weights through LDS
   fn spread_runs_2sbc[kMaxStaticM + 1];       // same, row weights by DPP row broadcast
   fn spread_runs_1w2sbc[kMaxStaticM + 1];     // one wavefront per tile, two streams, DPP
-  fn spread_runs2d[kMaxStaticM + 1];          // d = 2, float: run-merging, one wavefront
+  fn spread_runs2d[kMaxStaticM + 1];          // d = 2: run-merging, one wavefront
   fn spread_runs_1w3sbc[kMaxStaticM + 1];     // one wavefront per tile, three streams, DPP
   fn spread_runs_1w2s[kMaxStaticM + 1];       // one wavefront per tile, two streams
   fn spread_runs_1w3s[kMaxStaticM + 1];       // one wavefront per tile, three streams
@@ -1460,8 +1460,12 @@ void bk_fill_one(BKernelTable<R> &k) {
     k.spread_priv[D - 1][M_] = k_spread_wave_priv<R, D, 2 * M_ + 2>;
   }
   if constexpr (D == 3 && D * (2 * M_ + 2) <= 64) k.spread_rows[M_] = k_spread_rows2<R, 2 * M_ + 2, 2>;
-  if constexpr (std::is_same_v<R, float> && D == 2)
-    k.spread_runs2d[M_] = k_spread_runs2d<2 * M_ + 2, ((2 * M_ + 2) * (M_ + 1) > 64 ? 4 : 8), 2>;
+  if constexpr (D == 2) {
+    // FP64 factors take twice the registers: smaller node groups
+    constexpr int G2 = std::is_same_v<R, float> ? ((2 * M_ + 2) * (M_ + 1) > 64 ? 4 : 8)
+                                               : ((2 * M_ + 2) * (M_ + 1) > 64 ? 2 : 4);
+    k.spread_runs2d[M_] = k_spread_runs2d<R, 2 * M_ + 2, G2, 2>;
+  }
   if constexpr (std::is_same_v<R, float> && D == 3 && D * (2 * M_ + 2) <= 64) {
     k.spread_runs[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 1, false>;
     k.spread_runs_wide[M_] = k_spread_runs<2 * M_ + 2, runs_wide_nw(2 * M_ + 2), 8, 1, false>;

@@ -1,4 +1,4 @@
-// B^T for d = 2 in FP32 with same-cell runs summed in registers.
+// B^T for d = 2 (FP32 and FP64) with same-cell runs summed in registers.
 //
 // One wavefront per work item and LDS tile (E0 x E1 cells).  A lane owns one
 // tap of dimension 0 and a pair of adjacent taps of dimension 1 of the
@@ -11,9 +11,10 @@
 // in program order, so the streams' updates of shared cells do not race.
 //
 // Window values: G nodes at a time, each lane evaluates one tap (dimension
-// wt, tap ww) of one node as a per-tap polynomial, two nodes per pass when
-// 2 W <= 32.  Lanes of dimension 0 fold the node value in (psi0 f_j, complex),
-// and every lane fetches its three factors by ds_bpermute.
+// wt, tap ww) of one node, two nodes per pass when 2 W <= 32: FP32 as a
+// per-tap polynomial, FP64 in closed form.  Lanes of dimension 0 fold the
+// node value in (psi0 f_j, complex), and every lane fetches its factors by
+// ds_bpermute (two per FP64 value).
 #pragma once
 
 #include "engine.hpp"
@@ -21,15 +22,28 @@
 
 namespace nfft_hip {
 
-template <int W, int G, int S>
-__global__ __launch_bounds__(64) void k_spread_runs2d(const float2 *__restrict__ fin,
-                                                      float2 *__restrict__ g,
+// lane-to-lane fetch of a real value (ds_bpermute, byte address)
+template <typename R>
+__device__ __forceinline__ R bperm_r(int byte_addr, R v) {
+  if constexpr (sizeof(R) == 4) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(byte_addr, __float_as_int(v)));
+  } else {
+    const int lo = __builtin_amdgcn_ds_bpermute(byte_addr, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(byte_addr, __double2hiint(v));
+    return __hiloint2double(hi, lo);
+  }
+}
+
+template <typename R, int W, int G, int S>
+__global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict__ fin,
+                                                      cplx<R> *__restrict__ g,
                                                       const int *__restrict__ perm,
                                                       const int *__restrict__ cell,
-                                                      const float *__restrict__ frac,
+                                                      const R *__restrict__ frac,
                                                       const int4 *__restrict__ subs,
-                                                      BinGeom gg, WinParams<float> wp,
-                                                      float2 *__restrict__ tiles) {
+                                                      BinGeom gg, WinParams<R> wp,
+                                                      cplx<R> *__restrict__ tiles) {
+  using C = cplx<R>;
   static_assert(W % 2 == 0 && 2 * W <= 64, "W = 2m+2 taps");
   constexpr int HW = W / 2;
   constexpr int PAIRS = W * HW;
@@ -42,14 +56,14 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const float2 *__restrict__
   g += (long)blockIdx.y * gg.bs_g;
   if (tiles) tiles += (long)blockIdx.y * gg.bs_t;
   extern __shared__ unsigned char b_smem[];
-  float2 *tile = reinterpret_cast<float2 *>(b_smem);
+  C *tile = reinterpret_cast<C *>(b_smem);
   const int4 sp = subs[blockIdx.x];
   int lo[2];
   bin_corner<2>(sp.x, gg, lo);
   const int E1 = gg.E[1];
   const int tot = gg.E[0] * E1;
   const int lane = threadIdx.x;
-  for (int q = lane; q < tot; q += 64) tile[q] = float2{0.f, 0.f};
+  for (int q = lane; q < tot; q += 64) tile[q] = C{R(0), R(0)};
 
   // (dimension-0 tap, dimension-1 pair) slots of this lane
   int off[SLOTS];
@@ -75,11 +89,14 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const float2 *__restrict__
   const int wt = li / W < 2 ? li / W : 1;
   const int ww = li - (li / W) * W;
   float coef[kPolyCoef];
-  {
-    const float *pc = wp.poly + (wt * W + ww) * kPolyCoef;
+  if constexpr (sizeof(R) == 4) {
+    const R *pc = wp.poly + (wt * W + ww) * kPolyCoef;
 #pragma unroll
     for (int k = 0; k < kPolyCoef; k++) coef[k] = pc[k];
   }
+  // FP64: psi(frac + m - ww) of dimension wt in closed form
+  const R wb = wp.b[wt], wden = wp.inv_den[wt];
+  const R wm = R(W / 2 - 1);
 
   int sb[S], sn[S];
   {
@@ -90,11 +107,11 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const float2 *__restrict__
       sn[st] = min(sp.z, (st + 1) * part) - sb[st];
     }
   }
-  float2 acc[S][SLOTS][2];
+  C acc[S][SLOTS][2];
 #pragma unroll
   for (int st = 0; st < S; st++)
 #pragma unroll
-    for (int sl = 0; sl < SLOTS; sl++) acc[st][sl][0] = acc[st][sl][1] = float2{0.f, 0.f};
+    for (int sl = 0; sl < SLOTS; sl++) acc[st][sl][0] = acc[st][sl][1] = C{R(0), R(0)};
   int cur[S];
 #pragma unroll
   for (int st = 0; st < S; st++) cur[st] = -1;
@@ -102,7 +119,7 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const float2 *__restrict__
   auto flush = [&](int st) {
     const int corner = cur[st];
     // all reads, then all writes: the slots are distinct cells
-    float2 v[SLOTS][2];
+    C v[SLOTS][2];
 #pragma unroll
     for (int sl = 0; sl < SLOTS; sl++)
       if (pv[sl]) {
@@ -112,11 +129,10 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const float2 *__restrict__
 #pragma unroll
     for (int sl = 0; sl < SLOTS; sl++) {
       if (pv[sl]) {
-        tile[corner + off[sl]] = float2{v[sl][0].x + acc[st][sl][0].x, v[sl][0].y + acc[st][sl][0].y};
-        tile[corner + off[sl] + 1] =
-            float2{v[sl][1].x + acc[st][sl][1].x, v[sl][1].y + acc[st][sl][1].y};
+        tile[corner + off[sl]] = C{v[sl][0].x + acc[st][sl][0].x, v[sl][0].y + acc[st][sl][0].y};
+        tile[corner + off[sl] + 1] = C{v[sl][1].x + acc[st][sl][1].x, v[sl][1].y + acc[st][sl][1].y};
       }
-      acc[st][sl][0] = acc[st][sl][1] = float2{0.f, 0.f};
+      acc[st][sl][0] = acc[st][sl][1] = C{R(0), R(0)};
     }
   };
 
@@ -127,11 +143,11 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const float2 *__restrict__
     return (long)sp.y + (sn[st] > 0 ? sb[st] + (i < sn[st] ? i : 0) : 0);
   };
   int ckey[S];
-  float cfr[S][2];
-  float2 cf[S];
+  R cfr[S][2];
+  C cf[S];
   int nc[S][2];
-  float nfr[S][2];
-  float2 nf[S];
+  R nfr[S][2];
+  C nf[S];
   int nperm[S], nnperm[S];
   auto load_raw = [&](int st, int first, int pidx) {
     const long p = pos(st, first);
@@ -145,7 +161,8 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const float2 *__restrict__
   auto convert = [&](int st) {
     ckey[st] = (nc[st][0] - lo[0]) * E1 + (nc[st][1] - lo[1]);
 #pragma unroll
-    for (int t = 0; t < 2; t++) cfr[st][t] = nfr[st][t] - 0.5f;
+    // FP32: frac - 1/2 (the polynomials' variable); FP64: frac
+    for (int t = 0; t < 2; t++) cfr[st][t] = sizeof(R) == 4 ? nfr[st][t] - R(0.5) : nfr[st][t];
     cf[st] = nf[st];
   };
   int longest = 0;
@@ -171,32 +188,37 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const float2 *__restrict__
     }
     for (int base = 0; base < cmax; base += G) {
       // factors of nodes base .. base+G-1 of every stream
-      float ur[S][G][SLOTS], ui[S][G][SLOTS], v0[S][G][SLOTS], v1[S][G][SLOTS];
+      R ur[S][G][SLOTS], ui[S][G][SLOTS], v0[S][G][SLOTS], v1[S][G][SLOTS];
 #pragma unroll
       for (int st = 0; st < S; st++) {
 #pragma unroll
         for (int p = 0; p < NPASS; p++) {
           const int src = 4 * (base + NP * p + h);
-          const float f0 = bperm_f(src, cfr[st][0]);
-          const float f1 = bperm_f(src, cfr[st][1]);
-          const float fx = bperm_f(src, cf[st].x);
-          const float fy = bperm_f(src, cf[st].y);
-          const float tt = wt == 0 ? f0 : f1;
-          float v = coef[kPolyDeg];
+          const R f0 = bperm_r<R>(src, cfr[st][0]);
+          const R f1 = bperm_r<R>(src, cfr[st][1]);
+          const R fx = bperm_r<R>(src, cf[st].x);
+          const R fy = bperm_r<R>(src, cf[st].y);
+          const R tt = wt == 0 ? f0 : f1;
+          R v;
+          if constexpr (sizeof(R) == 4) {
+            v = coef[kPolyDeg];
 #pragma unroll
-          for (int k = kPolyDeg - 1; k >= 0; k--) v = fmaf(v, tt, coef[k]);
+            for (int k = kPolyDeg - 1; k >= 0; k--) v = fmaf(v, tt, coef[k]);
+          } else {
+            v = kb_psi<R>(tt + wm - R(ww), wm, wb, wden);
+          }
           // dimension-0 lanes carry psi0 f_j, dimension-1 lanes psi1
-          const float vr = wt == 0 ? v * fx : v;
-          const float vi = wt == 0 ? v * fy : v;
+          const R vr = wt == 0 ? v * fx : v;
+          const R vi = wt == 0 ? v * fy : v;
 #pragma unroll
           for (int hk = 0; hk < NP; hk++) {
             const int k = NP * p + hk;
 #pragma unroll
             for (int sl = 0; sl < SLOTS; sl++) {
-              ur[st][k][sl] = bperm_f(adr_u[hk][sl], vr);
-              ui[st][k][sl] = bperm_f(adr_u[hk][sl], vi);
-              v0[st][k][sl] = bperm_f(adr_v0[hk][sl], v);
-              v1[st][k][sl] = bperm_f(adr_v1[hk][sl], v);
+              ur[st][k][sl] = bperm_r<R>(adr_u[hk][sl], vr);
+              ui[st][k][sl] = bperm_r<R>(adr_u[hk][sl], vi);
+              v0[st][k][sl] = bperm_r<R>(adr_v0[hk][sl], v);
+              v1[st][k][sl] = bperm_r<R>(adr_v1[hk][sl], v);
             }
           }
         }
@@ -213,10 +235,10 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const float2 *__restrict__
             }
 #pragma unroll
             for (int sl = 0; sl < SLOTS; sl++) {
-              acc[st][sl][0].x = fmaf(ur[st][k][sl], v0[st][k][sl], acc[st][sl][0].x);
-              acc[st][sl][0].y = fmaf(ui[st][k][sl], v0[st][k][sl], acc[st][sl][0].y);
-              acc[st][sl][1].x = fmaf(ur[st][k][sl], v1[st][k][sl], acc[st][sl][1].x);
-              acc[st][sl][1].y = fmaf(ui[st][k][sl], v1[st][k][sl], acc[st][sl][1].y);
+              acc[st][sl][0].x = fma(ur[st][k][sl], v0[st][k][sl], acc[st][sl][0].x);
+              acc[st][sl][0].y = fma(ui[st][k][sl], v0[st][k][sl], acc[st][sl][0].y);
+              acc[st][sl][1].x = fma(ur[st][k][sl], v1[st][k][sl], acc[st][sl][1].x);
+              acc[st][sl][1].y = fma(ui[st][k][sl], v1[st][k][sl], acc[st][sl][1].y);
             }
           }
         }
@@ -234,7 +256,7 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const float2 *__restrict__
   for (int st = 0; st < S; st++)
     if (cur[st] >= 0) flush(st);
   __syncthreads();
-  flush_tile<float, 2>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, sp.w, lane, 64);
+  flush_tile<R, 2>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, sp.w, lane, 64);
 }
 
 }  // namespace nfft_hip
