@@ -153,6 +153,7 @@ template <typename R> struct Engine {
   int *bin_order = nullptr;  // zone reduction's bin order (inside sp_first's allocation)
   long bin_arrays_cap = 0;
   int spread_max_per_sub = 2048;
+  int spread_max_items = 0;  // spreading items per bin at most (0: no limit)
   // spreading tiles of all work items (no HBM atomics); null: atomic flush
   cplx<R> *tiles = nullptr;
   long tiles_cap = 0;

@@ -157,23 +157,33 @@ __global__ void k_node_place(BinGeom gg, const int *__restrict__ cell_tmp,
   }
 }
 
+// Work items of a bin: ceil(count / max_per_sub) of them, at most max_items
+// (0: no limit; dense bins then get longer items and fewer tiles), the
+// bin's nodes split evenly
+__device__ __forceinline__ int items_of(int cnt, int max_per_sub, int max_items) {
+  const int k = (cnt + max_per_sub - 1) / max_per_sub;
+  return max_items > 0 ? min(k, max_items) : k;
+}
+
 __global__ void k_count_subs(const int *__restrict__ hist, long nbins, int max_per_sub,
-                             int *__restrict__ nsub_of) {
+                             int max_items, int *__restrict__ nsub_of) {
   long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbins) return;
-  nsub_of[b] = (hist[b] + max_per_sub - 1) / max_per_sub;
+  nsub_of[b] = items_of(hist[b], max_per_sub, max_items);
 }
 
 __global__ void k_fill_subs(const int *__restrict__ hist, const int *__restrict__ bin_start,
                             const int *__restrict__ sub_start, long nbins, int max_per_sub,
-                            int4 *__restrict__ subs) {
+                            int max_items, int4 *__restrict__ subs) {
   long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbins) return;
   int cnt = hist[b];
   int s0 = sub_start[b];
-  for (int s = 0; s * max_per_sub < cnt; s++) {
-    int first = bin_start[b] + s * max_per_sub;
-    int c = min(max_per_sub, cnt - s * max_per_sub);
+  const int k = items_of(cnt, max_per_sub, max_items);
+  const int per = k > 0 ? (cnt + k - 1) / k : 0;
+  for (int s = 0; s < k; s++) {
+    int first = bin_start[b] + s * per;
+    int c = min(per, cnt - s * per);
     // .w: the item's place in bin order (its spreading tile's slot)
     subs[s0 + s] = make_int4((int)b, first, c, s0 + s);
   }
@@ -381,10 +391,10 @@ void bin_nodes(Engine<R> &e) {
   // work items of interpolation (small, for balance) and of spreading
   // (larger, to bound the number of tiles written to scratch); the spreading
   // list keeps its per-bin first item and count for the tile reduction
-  auto make_list = [&](int max_per_sub, int *first_of, int *count_of, int4 **subs, long *cap,
-                       long *nsub) {
+  auto make_list = [&](int max_per_sub, int max_items, int *first_of, int *count_of, int4 **subs,
+                       long *cap, long *nsub) {
     hipLaunchKernelGGL(k_count_subs, dim3((unsigned)ceil_div(nbins, 256)), dim3(256), 0, s, hist,
-                       nbins, max_per_sub, count_of);
+                       nbins, max_per_sub, max_items, count_of);
     hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, count_of, first_of, nbins,
                        sums + 1);
     int total = 0;
@@ -400,7 +410,7 @@ void bin_nodes(Engine<R> &e) {
     if (total > 0) {
       int4 *in_bin_order = *subs + total;
       hipLaunchKernelGGL(k_fill_subs, dim3((unsigned)ceil_div(nbins, 256)), dim3(256), 0, s, hist,
-                         bin_start, first_of, nbins, max_per_sub, in_bin_order);
+                         bin_start, first_of, nbins, max_per_sub, max_items, in_bin_order);
       // largest items first; key_of (M ints, total <= M) is free again
       int *bucket_of = key_of;
       int *bhist = sums + 2 + 3 * kOrderBuckets;
@@ -414,7 +424,7 @@ void bin_nodes(Engine<R> &e) {
                          in_bin_order, bucket_of, (long)total, boff, *subs);
     }
   };
-  make_list(e.max_per_sub, sub_start, nsub_of, &e.subs, &e.subs_cap, &e.nsub);
+  make_list(e.max_per_sub, 0, sub_start, nsub_of, &e.subs, &e.subs_cap, &e.nsub);
   // interpolation slots (pairs of same-cell nodes) for the kernels that use them
   e.geom.slots = e.geom.islots = nullptr;
   e.nslots = 0;
@@ -470,7 +480,8 @@ void bin_nodes(Engine<R> &e) {
   }
   e.sp_count = e.sp_first + nbins;
   e.bin_order = e.sp_first + 2 * nbins;
-  make_list(e.spread_max_per_sub, e.sp_first, e.sp_count, &e.sp_subs, &e.sp_subs_cap, &e.sp_nsub);
+  make_list(e.spread_max_per_sub, e.spread_max_items, e.sp_first, e.sp_count, &e.sp_subs,
+            &e.sp_subs_cap, &e.sp_nsub);
   // heavy bins first for the zone reduction (the interpolation list's
   // temporaries are free again: bucket per bin, bucket histogram and offsets)
   if (gg.zoned) {

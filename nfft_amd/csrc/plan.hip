@@ -236,6 +236,7 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   if (const char *v = std::getenv("NFFT_HIP_ITEM")) max_per_sub = std::max(64, std::atoi(v));
   if (const char *v = std::getenv("NFFT_HIP_SPREAD_ITEM"))
     spread_max_per_sub = std::max(64, std::atoi(v));
+  if (const char *v = std::getenv("NFFT_HIP_SPREAD_CAP")) spread_max_items = std::max(0, std::atoi(v));
   // B^T writes its LDS tiles to scratch and reduces them without atomics
   // when the tile fits LDS and never wraps onto itself (E_t <= n_t)
   {
