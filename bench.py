@@ -20,7 +20,8 @@ Also reported:
   stages        per stage: algorithmic HBM bytes and GB/s against 8 TB/s;
                 for B and B^T also the on-chip rates that bound them: taps
                 (2m+2)^d M per launch, 8 bytes of LDS grid data per tap
-                (interpolation reads one complex value per tap) against
+                (algorithmic: one complex value per tap; the 3-D kernel's
+                same-cell node pairs read about 55% of that) against
                 256 CUs x 128 B/clk x 2.4 GHz = 78.6 TB/s, and 4 flops per tap
                 (complex value times real weight, two FMAs) against the
                 78.6 TFLOP/s of unpacked FP32 FMA issue (256 CUs x 4 SIMDs x
