@@ -365,9 +365,22 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
 template <typename R> constexpr int fft_cap() { return sizeof(R) == 4 ? 4096 : 2048; }
 
 template <typename R>
-void launch_fft_pass(cplx<R> *g, const cplx<R> *fhat_in, cplx<R> *fhat_out, const FftPass<R> &p,
+void launch_fft_pass(cplx<R> *g, const cplx<R> *fhat_in, cplx<R> *fhat_out, const FftPass<R> &p0,
                      int sign, hipStream_t s, int nb) {
-  if (p.nlines <= 0) return;
+  if (p0.nlines <= 0) return;
+  // Fewer lines per workgroup than fit in LDS when the launch would leave
+  // the GPU short of workgroups (small grids: aim for 1024 of them over all
+  // right-hand sides), keeping at least 2 lines (4 along a strided axis, so
+  // that each load covers 32 contiguous bytes or more).
+  FftPass<R> p = p0;
+  {
+    const long fit = p0.lines_per_block;
+    const long want = std::max<long>(1, p0.nlines * (long)nb / 1024);
+    const long floor_lines = std::min<long>(fit, p0.stride == 1 ? 2 : 4);
+    p.lines_per_block = (int)std::max(floor_lines, std::min(fit, want));
+    const size_t line_bytes = sizeof(cplx<R>) * (size_t)p.lines_per_block * lds_line_stride(p.len);
+    p.twiddle_in_lds = line_bytes + sizeof(cplx<R>) * (size_t)p.len <= 64 * 1024 ? 1 : 0;
+  }
   const long blocks = ceil_div(p.nlines, p.lines_per_block);
   const size_t lds = sizeof(cplx<R>) * ((size_t)p.lines_per_block * (size_t)lds_line_stride(p.len) +
                                         (p.twiddle_in_lds ? (size_t)p.len : 0));

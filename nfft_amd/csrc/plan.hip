@@ -412,6 +412,8 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
       }
       p.fstr_ax = fstride[t];
       p.phi_ax = phi_inv + phioff[t];
+      // lines per workgroup that fit the LDS budget (the launch may use
+      // fewer, see launch_fft_pass)
       p.lines_per_block = (int)std::max<long>(1, std::min<long>(256, cap / n[t]));
       p.nstages = axes[t].nstages;
       for (int i = 0; i < p.nstages; i++) p.radix[i] = axes[t].radix[i];
