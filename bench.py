@@ -155,14 +155,17 @@ def cgnr_leg(args):
         res = {"trajectory": list(desc), "M": int(p.M)}
         traces = {}
         for name in names:
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            w = None
-            if name == "pipe_menon":
-                w = D.pipe_menon(p, 20)
-            elif name == "learned":
-                w = model.weights(p).cpu().numpy()
-            t_w = time.perf_counter() - t0
+            # weight computation time: median of 3 (device work and the copy
+            # of w to the host, where the solver's W lives)
+            w, tws = None, [0.0]
+            if name != "plain":
+                tws = []
+                for _ in range(3):
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    w = D.pipe_menon(p, 20) if name == "pipe_menon" else model.weights(p).cpu().numpy()
+                    tws.append(time.perf_counter() - t0)
+            t_w = float(np.median(tws))
             it, t_s, r, _ = T.cgnr_time_to(p, y0, w, tol=tol, max_iter=args.cgnr_max_iter)
             traces[name] = T.image_error_trace(p, y, f_true, w, 300)
             # gridding: one weighted adjoint of the noisy data

@@ -139,44 +139,60 @@ __device__ __forceinline__ int coarse_cell(float x, int G) {
   return c < 0 ? 0 : (c >= G ? G - 1 : c);
 }
 
-__global__ void k_dcf_hist(const float *__restrict__ x, long M, int *__restrict__ counts) {
-  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= M) return;
-  int off = 0;
-#pragma unroll
-  for (int s = 0; s < kDcfScales; s++) {
-    const int G = c_dcf_grid[s];
-    const int cx = coarse_cell(x[2 * j], G), cy = coarse_cell(x[2 * j + 1], G);
-    atomicAdd(&counts[off + cx * G + cy], 1);
-    off += G * G;
+// Coarse-grid node counts.  Dense trajectories put thousands of nodes in one
+// coarse cell, so the two coarse grids (32^2 + 64^2 cells) are counted in a
+// per-workgroup LDS histogram first (one global add per touched cell and
+// workgroup); the 128^2 grid, whose cells see far fewer nodes, directly.
+constexpr int kDcfHistNodes = 4096;  // nodes per workgroup
+__global__ __launch_bounds__(256) void k_dcf_hist(const float *__restrict__ x, long M,
+                                                  int *__restrict__ counts) {
+  constexpr int L = 32 * 32 + 64 * 64;
+  __shared__ int h[L];
+  for (int q = threadIdx.x; q < L; q += 256) h[q] = 0;
+  __syncthreads();
+  const long j0 = (long)blockIdx.x * kDcfHistNodes;
+  const long j1 = min(M, j0 + kDcfHistNodes);
+  for (long j = j0 + threadIdx.x; j < j1; j += 256) {
+    const float x0 = x[2 * j], x1 = x[2 * j + 1];
+    atomicAdd(&h[coarse_cell(x0, 32) * 32 + coarse_cell(x1, 32)], 1);
+    atomicAdd(&h[1024 + coarse_cell(x0, 64) * 64 + coarse_cell(x1, 64)], 1);
+    atomicAdd(&counts[L + coarse_cell(x0, 128) * 128 + coarse_cell(x1, 128)], 1);
   }
+  __syncthreads();
+  for (int q = threadIdx.x; q < L; q += 256)
+    if (h[q]) atomicAdd(&counts[q], h[q]);
 }
 
-// rows j0 .. j0+n-1 of the features into feat[0 .. n-1]
-__global__ void k_dcf_features(const float *__restrict__ x, long M, long j0, long n,
-                               const int *__restrict__ counts, float *__restrict__ feat) {
-  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
+// rows j0 .. j0+n-1 of the features into feat[0 .. n-1], one thread per
+// (row, feature) so that the row-major stores are coalesced; the density
+// estimate column from v = Re(B B^T 1) and its sum *total
+__global__ __launch_bounds__(256) void k_dcf_features_flat(const float *__restrict__ x, long M,
+                                                           long j0, long n,
+                                                           const int *__restrict__ counts,
+                                                           const float2 *__restrict__ v,
+                                                           const double *__restrict__ total,
+                                                           float *__restrict__ feat) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * kDcfFeat) return;
+  const long r = i / kDcfFeat;
+  const int q = (int)(i - r * kDcfFeat);
   const long j = j0 + r;
   const float x0 = x[2 * j], x1 = x[2 * j + 1];
-  float *fj = feat + r * kDcfFeat;
-  fj[0] = 2.f * sqrtf(x0 * x0 + x1 * x1);
-  int off = 0, q = 1;
-#pragma unroll
-  for (int s = 0; s < kDcfScales; s++) {
+  float val = 0.f;
+  if (q == 0) {
+    val = 2.f * sqrtf(x0 * x0 + x1 * x1);
+  } else if (q < 1 + kDcfScales * 25) {
+    const int s = (q - 1) / 25, ab = (q - 1) - s * 25;
     const int G = c_dcf_grid[s];
-    const float inv_cbar = (float)G * (float)G / (float)M;
-    const int cx = coarse_cell(x0, G), cy = coarse_cell(x1, G);
-    for (int a = -2; a <= 2; a++) {
-      const int ux = (cx + a + G) % G;
-      for (int b = -2; b <= 2; b++) {
-        const int uy = (cy + b + G) % G;
-        fj[q++] = log1pf((float)counts[off + ux * G + uy] * inv_cbar);
-      }
-    }
-    off += G * G;
+    const int off = s == 0 ? 0 : (s == 1 ? 32 * 32 : 32 * 32 + 64 * 64);
+    const int ux = (coarse_cell(x0, G) + ab / 5 - 2 + G) % G;
+    const int uy = (coarse_cell(x1, G) + ab % 5 - 2 + G) % G;
+    val = log1pf((float)counts[off + ux * G + uy] * ((float)G * (float)G / (float)M));
+  } else if (q == kDcfRaw - 1) {
+    const float mean = (float)(total[0] / (double)M);
+    val = logf(fmaxf(v[j].x, 1e-30f) / mean);
   }
-  for (; q < kDcfFeat; q++) fj[q] = 0.f;
+  feat[i] = val;
 }
 
 __global__ void k_fill_unit(float2 *__restrict__ out, long M) {
@@ -212,14 +228,6 @@ __global__ __launch_bounds__(256) void k_sum_parts(const double *__restrict__ pa
   if (threadIdx.x == 0) total[0] = red[0];
 }
 
-__global__ void k_dcf_kde(const float2 *__restrict__ v, long M, long j0, long n,
-                          const double *__restrict__ total, float *__restrict__ feat) {
-  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  const float mean = (float)(total[0] / (double)M);
-  feat[r * kDcfFeat + kDcfRaw - 1] = logf(fmaxf(v[j0 + r].x, 1e-30f) / mean);
-}
-
 // The parts of the features shared by all rows, enqueued on the plan's
 // stream: coarse-grid counts, and the density estimate v = Re(B B^T 1) with
 // its FP64 sum at part[nb]
@@ -228,7 +236,8 @@ static void features_common(Engine<float> *e, const DcfScratch &sc) {
   hipStream_t s = e->stream;
   const long nb = ceil_div(M, 256);
   NFFT_HIP_CHECK(float, hipMemsetAsync(sc.counts, 0, sizeof(int) * kDcfCells, s));
-  hipLaunchKernelGGL(k_dcf_hist, dim3(nblk(M)), dim3(256), 0, s, e->xdev, M, sc.counts);
+  hipLaunchKernelGGL(k_dcf_hist, dim3((unsigned)ceil_div(M, kDcfHistNodes)), dim3(256), 0, s,
+                     e->xdev, M, sc.counts);
   hipLaunchKernelGGL(k_fill_unit, dim3(nblk(M)), dim3(256), 0, s, sc.fa, M);
   e->spread_to_grid(sc.fa);
   e->interp_from_grid(sc.fb);
@@ -240,10 +249,8 @@ static void features_common(Engine<float> *e, const DcfScratch &sc) {
 static void features_rows(Engine<float> *e, const DcfScratch &sc, long j0, long n, float *feat) {
   const long M = e->M;
   hipStream_t s = e->stream;
-  hipLaunchKernelGGL(k_dcf_features, dim3(nblk(n)), dim3(256), 0, s, e->xdev, M, j0, n, sc.counts,
-                     feat);
-  hipLaunchKernelGGL(k_dcf_kde, dim3(nblk(n)), dim3(256), 0, s, sc.fb, M, j0, n,
-                     sc.part + ceil_div(M, 256), feat);
+  hipLaunchKernelGGL(k_dcf_features_flat, dim3(nblk(n * kDcfFeat)), dim3(256), 0, s, e->xdev, M,
+                     j0, n, sc.counts, sc.fb, sc.part + ceil_div(M, 256), feat);
 }
 
 // all M feature rows into the caller's feat
