@@ -30,8 +30,8 @@ Also reported:
                 HBM figure includes the tile reduction, its on-chip figures
                 cover the spreading kernel alone
   cpu_baseline  the FP64 CPU oracle on a bounded sample of the same workload
-                (full-grid D + FFT, B on 2^18 of the nodes, scaled to M),
-                single-threaded on the host
+                (full-grid D + FFT, B on 2^22 of the nodes, scaled to M), on
+                up to 16 host cores (OpenMP)
   batched       32 coil right-hand sides on one 2-D plan and 64 plans of
                 uneven size, eager and as hipGraph replays
   other_configs the 1-D FP64 and 2-D (512^2, FP32 / FP64, m 4 / 6) single-plan
@@ -78,10 +78,12 @@ def algorithmic_bytes(N, n, M, d, c, r):
     return dict(D=D, F=F, B=B, DT=DT, trafo=D + F + B, adjoint=B + F + DT)
 
 
-def cpu_baseline(N, n, m, x, fh, f, sample):
-    """FP64 oracle on the host: full-grid D + FFT both ways, B and B^T on
-    `sample` nodes, B time scaled to all nodes."""
+def cpu_baseline(N, n, m, x, fh, f, sample, threads):
+    """FP64 oracle on the host (its FFT and B / B^T loops on `threads` OpenMP
+    threads): full-grid D + FFT both ways, B and B^T on `sample` nodes, B
+    time scaled to all nodes."""
     import oracle as O
+    O.set_threads(threads)
     M = x.shape[0]
     idx = np.random.default_rng(5).choice(M, size=sample, replace=False)
     xs = np.ascontiguousarray(x[idx])
@@ -96,13 +98,15 @@ def cpu_baseline(N, n, m, x, fh, f, sample):
     gt = O.fft(n, g, +1)
     O.stage_DT(N, n, m, gt)
     t4 = time.perf_counter()
+    O.set_threads(1)
     fixed = (t1 - t0) + (t4 - t3)
     per_node = ((t2 - t1) + (t3 - t2)) / sample
     est = fixed + per_node * M
-    return dict(value=M / est, unit="nodes/s", cores=1, kind="port",
-                sample="oracle (FP64, 1 thread) on the same workload: full 256^3 D+FFT for trafo "
-                       "and adjoint, B and B^T on %d random nodes of the kooshball, B time scaled "
-                       "to M=%d; measured %.1f s of CPU work" % (sample, M, t4 - t0))
+    return dict(value=M / est, unit="nodes/s", cores=threads, kind="port",
+                sample="oracle (FP64, %d OpenMP threads; the reference itself needs FFTW, absent "
+                       "here) on the same workload: full 256^3 D+FFT for trafo and adjoint, B and "
+                       "B^T on %d random nodes of the kooshball, B time scaled to M=%d; measured "
+                       "%.1f s wall" % (threads, sample, M, t4 - t0))
 
 
 def cgnr_leg(args):
@@ -376,7 +380,9 @@ def main():
     ap.add_argument("--no-cgnr", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cgnr-max-iter", type=int, default=1000)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 18)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="nodes of the CPU baseline's B / B^T sample (0: 2^22 with 8 or more "
+                         "threads, else 2^18)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes "
                          "(calibrates the HBM byte counters of a profiler run)")
@@ -499,7 +505,9 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, bounded sample) ...")
-        cpu = cpu_baseline(N, n, m, x, fh, f_in, min(args.cpu_sample, M))
+        threads = min(16, len(os.sched_getaffinity(0)))
+        sample = min(args.cpu_sample if args.cpu_sample else (1 << 22 if threads >= 8 else 1 << 18), M)
+        cpu = cpu_baseline(N, n, m, x, fh, f_in, sample, threads)
         log("cpu baseline %.3g nodes/s" % cpu["value"])
 
     out = {

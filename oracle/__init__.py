@@ -52,6 +52,8 @@ def lib():
         for name in ("oracle_ndft_trafo", "oracle_ndft_adjoint"):
             getattr(L, name).argtypes = [ctypes.c_int, lp, ctypes.c_long, dp, dp, dp]
         L.oracle_fft.argtypes = [ctypes.c_int, lp, dp, ctypes.c_int]
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_get_threads.restype = ctypes.c_int
         for name in ("oracle_D", "oracle_DT"):
             getattr(L, name).argtypes = [ctypes.c_int, lp, lp, ctypes.c_long, dp, dp]
         for name in ("oracle_B", "oracle_BT", "oracle_nfft_trafo", "oracle_nfft_adjoint"):
@@ -82,6 +84,11 @@ def _x(x, d):
 
 def bessel_i0(x):
     return lib().oracle_bessel_i0(float(x))
+
+
+def set_threads(n):
+    """Threads of the oracle's FFT and B / B^T loops (1 by default)."""
+    lib().oracle_set_threads(int(n))
 
 
 def phi(n, x, m, b):
