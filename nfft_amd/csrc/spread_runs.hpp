@@ -163,10 +163,10 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
     for (int j = 0; j < RW; j++)
 #pragma unroll
       for (int sl = 0; sl < SLOTS; sl++) acc[st][j][sl][0] = acc[st][j][sl][1] = float2{0.f, 0.f};
-  // current run per stream: packed tile corner | (row a0 << 20); -1 when none
+  // current run per stream: packed tile corner | (row a0 << 20), from the
+  // stream's first node on (a stream without nodes holds the item's first
+  // node's corner and only ever adds zeros there)
   int cur[S];
-#pragma unroll
-  for (int st = 0; st < S; st++) cur[st] = -1;
   auto flush = [&](int st) {
     const int corner = cur[st] & 0xfffff;
     const int a0 = cur[st] >> 20;
@@ -198,11 +198,14 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
   // value.  Loads run two chunks ahead for the node index (perm) and one
   // chunk ahead for everything else, and are converted only when the chunk
   // becomes current, so no wait for memory falls inside a chunk's work.
+  // Lanes past the stream's end repeat its last node with value zero: they
+  // stay in the last run and add nothing, so the node loop needs no bounds
+  // test per node.
   auto pos = [&](int st, int first) {
-    // a valid node of the work item for every lane (empty streams and lanes
-    // past the stream's end read the stream's or the item's first node)
+    // a valid node of the work item for every lane (an empty stream reads
+    // the item's first node)
     const int i = first + lane;
-    return (long)sp.y + (sn[st] > 0 ? sb[st] + (i < sn[st] ? i : 0) : 0);
+    return (long)sp.y + (sn[st] > 0 ? sb[st] + (i < sn[st] ? i : sn[st] - 1) : 0);
   };
   int ckey[S];
   float cfr[S][3];
@@ -220,7 +223,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
     }
     nf[st] = fin[pidx];
   };
-  auto convert = [&](int st) {
+  auto convert = [&](int st, int first) {
     int a[3];
 #pragma unroll
     for (int t = 0; t < 3; t++) {
@@ -228,7 +231,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
       cfr[st][t] = nfr[st][t] - 0.5f;
     }
     ckey[st] = ((a[0] * E1 + a[1]) * E2 + a[2]) | (a[0] << 20);
-    cf[st] = nf[st];
+    cf[st] = first + lane < sn[st] ? nf[st] : float2{0.f, 0.f};
   };
   int longest = 0;
 #pragma unroll
@@ -236,7 +239,8 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
     load_raw(st, 0, perm[pos(st, 0)]);
     nperm[st] = perm[pos(st, 64)];
     nnperm[st] = 0;
-    convert(st);
+    convert(st, 0);
+    cur[st] = __builtin_amdgcn_readfirstlane(ckey[st]);
     longest = max(longest, sn[st]);
   }
   for (int first = 0; first < longest; first += 64) {
@@ -317,27 +321,25 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
 #pragma unroll
         for (int st = 0; st < S; st++) {
           if constexpr (BC) {
-            if (base + k < cnt[st]) {
-              const int key = gkey[st][k];
-              float a0[SLOTS], a1[SLOTS];
+            const int key = gkey[st][k];
+            float a0[SLOTS], a1[SLOTS];
 #pragma unroll
-              for (int sl = 0; sl < SLOTS; sl++) {
-                a0[sl] = w0[st][k][sl];
-                a1[sl] = w1[st][k][sl];
-              }
-              if (key != cur[st]) {
-                if (cur[st] >= 0) flush(st);
-                cur[st] = key;
-              }
-              rows_bcast<0, RW, SLOTS>(acc[st], brr[st][k], bri[st][k], a0, a1);
+            for (int sl = 0; sl < SLOTS; sl++) {
+              a0[sl] = w0[st][k][sl];
+              a1[sl] = w1[st][k][sl];
             }
+            if (key != cur[st]) {
+              flush(st);
+              cur[st] = key;
+            }
+            rows_bcast<0, RW, SLOTS>(acc[st], brr[st][k], bri[st][k], a0, a1);
             continue;
           }
-          if (base + k < cnt[st]) {
+          {
             const int p = k / NP, hk = k % NP;
             const int key = __builtin_amdgcn_readlane(ckey[st], base + k);
             if (key != cur[st]) {
-              if (cur[st] >= 0) flush(st);
+              flush(st);
               cur[st] = key;
             }
             const int r0 = ((wv - (key >> 20)) % NW + NW) % NW;
@@ -366,14 +368,17 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
 #pragma unroll
     for (int st = 0; st < S; st++) {
       if (first + 64 < sn[st]) {
-        convert(st);
+        convert(st, first + 64);
         nperm[st] = nnperm[st];
+      } else {
+        // stream done: zero-valued copies of its last node (its open run)
+        ckey[st] = __builtin_amdgcn_readlane(ckey[st], 63);
+        cf[st] = float2{0.f, 0.f};
       }
     }
   }
 #pragma unroll
-  for (int st = 0; st < S; st++)
-    if (cur[st] >= 0) flush(st);
+  for (int st = 0; st < S; st++) flush(st);
   __syncthreads();
   flush_tile<float, 3>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, sp.w, threadIdx.x,
                        blockDim.x);
