@@ -112,9 +112,8 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
   for (int st = 0; st < S; st++)
 #pragma unroll
     for (int sl = 0; sl < SLOTS; sl++) acc[st][sl][0] = acc[st][sl][1] = C{R(0), R(0)};
+  // current run per stream (tile corner), from the stream's first node on
   int cur[S];
-#pragma unroll
-  for (int st = 0; st < S; st++) cur[st] = -1;
   __syncthreads();
   auto flush = [&](int st) {
     const int corner = cur[st];
@@ -136,11 +135,13 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
     }
   };
 
-  // chunk of 64 nodes per stream, one per lane (a valid node for every lane;
-  // lanes past the stream's end repeat its first node and are never used)
+  // chunk of 64 nodes per stream, one per lane (a valid node for every lane:
+  // lanes past the stream's end repeat its last node with value zero, so they
+  // stay in the last run and add nothing; an empty stream reads the item's
+  // first node)
   auto pos = [&](int st, int first) {
     const int i = first + lane;
-    return (long)sp.y + (sn[st] > 0 ? sb[st] + (i < sn[st] ? i : 0) : 0);
+    return (long)sp.y + (sn[st] > 0 ? sb[st] + (i < sn[st] ? i : sn[st] - 1) : 0);
   };
   int ckey[S];
   R cfr[S][2];
@@ -158,12 +159,12 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
     }
     nf[st] = fin[pidx];
   };
-  auto convert = [&](int st) {
+  auto convert = [&](int st, int first) {
     ckey[st] = (nc[st][0] - lo[0]) * E1 + (nc[st][1] - lo[1]);
 #pragma unroll
     // FP32: frac - 1/2 (the polynomials' variable); FP64: frac
     for (int t = 0; t < 2; t++) cfr[st][t] = sizeof(R) == 4 ? nfr[st][t] - R(0.5) : nfr[st][t];
-    cf[st] = nf[st];
+    cf[st] = first + lane < sn[st] ? nf[st] : C{R(0), R(0)};
   };
   int longest = 0;
 #pragma unroll
@@ -171,7 +172,8 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
     load_raw(st, 0, perm[pos(st, 0)]);
     nperm[st] = perm[pos(st, 64)];
     nnperm[st] = 0;
-    convert(st);
+    convert(st, 0);
+    cur[st] = __builtin_amdgcn_readfirstlane(ckey[st]);
     longest = max(longest, sn[st]);
   }
   for (int first = 0; first < longest; first += 64) {
@@ -227,19 +229,17 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
       for (int k = 0; k < G; k++) {
 #pragma unroll
         for (int st = 0; st < S; st++) {
-          if (base + k < cnt[st]) {
-            const int key = __builtin_amdgcn_readlane(ckey[st], base + k);
-            if (key != cur[st]) {
-              if (cur[st] >= 0) flush(st);
-              cur[st] = key;
-            }
+          const int key = __builtin_amdgcn_readlane(ckey[st], base + k);
+          if (key != cur[st]) {
+            flush(st);
+            cur[st] = key;
+          }
 #pragma unroll
-            for (int sl = 0; sl < SLOTS; sl++) {
-              acc[st][sl][0].x = fma(ur[st][k][sl], v0[st][k][sl], acc[st][sl][0].x);
-              acc[st][sl][0].y = fma(ui[st][k][sl], v0[st][k][sl], acc[st][sl][0].y);
-              acc[st][sl][1].x = fma(ur[st][k][sl], v1[st][k][sl], acc[st][sl][1].x);
-              acc[st][sl][1].y = fma(ui[st][k][sl], v1[st][k][sl], acc[st][sl][1].y);
-            }
+          for (int sl = 0; sl < SLOTS; sl++) {
+            acc[st][sl][0].x = fma(ur[st][k][sl], v0[st][k][sl], acc[st][sl][0].x);
+            acc[st][sl][0].y = fma(ui[st][k][sl], v0[st][k][sl], acc[st][sl][0].y);
+            acc[st][sl][1].x = fma(ur[st][k][sl], v1[st][k][sl], acc[st][sl][1].x);
+            acc[st][sl][1].y = fma(ui[st][k][sl], v1[st][k][sl], acc[st][sl][1].y);
           }
         }
       }
@@ -247,14 +247,17 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
 #pragma unroll
     for (int st = 0; st < S; st++) {
       if (first + 64 < sn[st]) {
-        convert(st);
+        convert(st, first + 64);
         nperm[st] = nnperm[st];
+      } else {
+        // stream done: zero-valued copies of its last node (its open run)
+        ckey[st] = __builtin_amdgcn_readlane(ckey[st], 63);
+        cf[st] = C{R(0), R(0)};
       }
     }
   }
 #pragma unroll
-  for (int st = 0; st < S; st++)
-    if (cur[st] >= 0) flush(st);
+  for (int st = 0; st < S; st++) flush(st);
   __syncthreads();
   flush_tile<R, 2>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, sp.w, lane, 64);
 }
