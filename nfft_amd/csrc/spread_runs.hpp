@@ -163,14 +163,14 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
     for (int j = 0; j < RW; j++)
 #pragma unroll
       for (int sl = 0; sl < SLOTS; sl++) acc[st][j][sl][0] = acc[st][j][sl][1] = float2{0.f, 0.f};
-  // current run per stream: packed tile corner | (row a0 << 20), from the
+  // current run per stream: packed key (tile corner | 4 r0 << 20, r0 the
+  // first of the node's window rows that this wavefront owns), from the
   // stream's first node on (a stream without nodes holds the item's first
   // node's corner and only ever adds zeros there)
   int cur[S];
   auto flush = [&](int st) {
     const int corner = cur[st] & 0xfffff;
-    const int a0 = cur[st] >> 20;
-    const int r0 = ((wv - a0) % NW + NW) % NW;
+    const int r0 = cur[st] >> 22;
     // all reads, then all writes: the rows are distinct cells
 #pragma unroll
     for (int sl = 0; sl < SLOTS; sl++) {
@@ -230,7 +230,9 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
       a[t] = nc[st][t] - lo[t];
       cfr[st][t] = nfr[st][t] - 0.5f;
     }
-    ckey[st] = ((a[0] * E1 + a[1]) * E2 + a[2]) | (a[0] << 20);
+    // the corner determines r0, so keys of one wavefront compare as corners
+    const int r0 = ((wv - a[0]) % NW + NW) % NW;
+    ckey[st] = ((a[0] * E1 + a[1]) * E2 + a[2]) | (r0 << 22);
     cf[st] = first + lane < sn[st] ? nf[st] : float2{0.f, 0.f};
   };
   int longest = 0;
@@ -280,8 +282,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
           xi[st][p] = v * fy;
           if constexpr (TAB) {
             // lanes holding a row tap of this wavefront's rows store its weight
-            const int a0 = __builtin_amdgcn_ds_bpermute(src, ckey[st]) >> 20;
-            const int r0n = ((wv - a0) % NW + NW) % NW;
+            const int r0n = __builtin_amdgcn_ds_bpermute(src, ckey[st]) >> 22;
             const int d = ww - r0n;
             if (wt == 0 && li < W && d >= 0 && d % NW == 0)
               rtab[(st * G + NP * p + h) * RW + d / NW] = float2{xr[st][p], xi[st][p]};
@@ -310,8 +311,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
           for (int k = 0; k < G; k++) {
             const int p = k / NP, hk = k % NP;
             gkey[st][k] = __builtin_amdgcn_readlane(ckey[st], base + k);
-            const int r0 = ((wv - (gkey[st][k] >> 20)) % NW + NW) % NW;
-            const int src = 4 * (hk * 32 + r0) + qoff;
+            const int src = (gkey[st][k] >> 20) + 4 * hk * 32 + qoff;
             brr[st][k] = bperm_f(src, xr[st][p]);
             bri[st][k] = bperm_f(src, xi[st][p]);
           }
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
               flush(st);
               cur[st] = key;
             }
-            const int r0 = ((wv - (key >> 20)) % NW + NW) % NW;
+            const int r0 = key >> 22;
             float2 rw[RW];
             if constexpr (TAB) {
 #pragma unroll
