@@ -182,7 +182,8 @@ template <typename R> struct Engine {
   // spreading also 2 private tile per wavefront (priv), 3 row-partitioned
   // tile (rows), and the FP32 3-D run-merging kernels 4 (runs), 5 (runs5),
   // 6 (runs2s), 7 (runs2st), 8 (runs1w2s), 9 (runs1w3s), 10 (runs2sbc),
-  // 11 (runs1w2sbc), 12 (runs1w3sbc).
+  // 11 (runs1w2sbc), 12 (runs1w3sbc), 13 (runs2rs: one stream, runs load and
+  // store their tile cells), and the 2-D run-merging kernel 14 (runs2d).
   // Set at plan creation; NFFT_HIP_INTERP / NFFT_HIP_SPREAD = name override.
   int interp_variant = 0;
   int spread_variant = 1;

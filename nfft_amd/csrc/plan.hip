@@ -324,6 +324,7 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     if (!v) return dflt;
     return std::strcmp(v, "runs1w3s") == 0  ? 9
            : std::strcmp(v, "runs2sbc") == 0 ? 10
+           : std::strcmp(v, "runs2rs") == 0 ? 13
            : std::strcmp(v, "runs1w2sbc") == 0 ? 11
            : std::strcmp(v, "runs1w3sbc") == 0 ? 12
            : std::strcmp(v, "runs2d") == 0 ? 14
@@ -646,7 +647,7 @@ void Engine<R>::autotune() {
     // float candidates accumulate in plain FP32 (the fixed-point LDS
     // variants are left out); double ones use ds_add_f64
     std::vector<int> cand;
-    if (sizeof(R) == 4 && d == 3) cand = {10, 11, 6, 8, 9, 7, 4, 5, 3};
+    if (sizeof(R) == 4 && d == 3) cand = {10, 13, 11, 6, 8, 9, 7, 4, 5, 3};
     else if (sizeof(R) == 4 && d == 2) cand = {14, 1, 0, 2};
     else if (sizeof(R) == 8 && d == 2) cand = {14, 1, 0};
     else if (sizeof(R) == 8 && d >= 2) cand = {1, 0};

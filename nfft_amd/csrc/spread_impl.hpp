@@ -1436,6 +1436,7 @@ template <typename R> struct BKernelTable {
   fn spread_runs_2s[kMaxStaticM + 1];         // same, two node streams per wavefront
   fn spread_runs_2st[kMaxStaticM + 1];        // same, row weights through LDS
   fn spread_runs_2sbc[kMaxStaticM + 1];       // same, row weights by DPP row broadcast
+  fn spread_runs_2rs[kMaxStaticM + 1];        // two wavefronts, one stream, run load-store
   fn spread_runs_1w2sbc[kMaxStaticM + 1];     // one wavefront per tile, two streams, DPP
   fn spread_runs2d[kMaxStaticM + 1];          // d = 2: run-merging, one wavefront
   fn spread_runs_1w3sbc[kMaxStaticM + 1];     // one wavefront per tile, three streams, DPP
@@ -1472,6 +1473,7 @@ void bk_fill_one(BKernelTable<R> &k) {
     k.spread_runs_2s[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false>;
     k.spread_runs_2st[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, true>;
     k.spread_runs_2sbc[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false, true>;
+    k.spread_runs_2rs[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 1, false, true, true>;
     if constexpr (2 * M_ + 2 <= 16) {  // all rows' weights in one 16-lane DPP row
       k.spread_runs_1w2sbc[M_] = k_spread_runs<2 * M_ + 2, 1, 8, 2, false, true>;
       k.spread_runs_1w3sbc[M_] = k_spread_runs<2 * M_ + 2, 1, 8, 3, false, true>;
@@ -1542,6 +1544,10 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
     if (spread && variant == 10 && gg.d == 3 && k.spread_runs_2sbc[gg.m]) {
       threads = 128;
       return k.spread_runs_2sbc[gg.m];
+    }
+    if (spread && variant == 13 && gg.d == 3 && k.spread_runs_2rs[gg.m]) {
+      threads = 128;
+      return k.spread_runs_2rs[gg.m];
     }
     if (spread && variant == 7 && gg.d == 3 && k.spread_runs_2st[gg.m]) {
       threads = 128;
@@ -1627,7 +1633,9 @@ bool batch_kernels(const Engine<R> &e) {
     const BinGeom &gg = e.geom;
     if (!(e.tile_scratch && gg.zoned && gg.m >= 1 && gg.m <= kMaxStaticM)) return false;
     if (tile_bytes<R>(gg) > kTileBudget || e.interp_variant != 0) return false;
-    if (gg.d == 3) return e.spread_variant == 10 && bkernels<R>().spread_runs_2sbc[gg.m];
+    if (gg.d == 3)
+      return (e.spread_variant == 10 && bkernels<R>().spread_runs_2sbc[gg.m]) ||
+             (e.spread_variant == 13 && bkernels<R>().spread_runs_2rs[gg.m]);
     if (gg.d == 2) return e.spread_variant == 14 && bkernels<R>().spread_runs2d[gg.m];
     return false;
   }

@@ -78,7 +78,12 @@ constexpr int runs_wide_nw(int W) {
 // lanes 0..RW-1 of every 16-lane row and the FMAs read them with a DPP row
 // broadcast, replacing 2 RW readlanes per node (and their scalar index
 // arithmetic) by two ds_bpermute.
-template <int W, int NW, int G, int S, bool TAB, bool BC = false>
+// RS (one stream): a run loads its cells of the tile into the accumulators
+// when it starts and stores them back when it ends, with no add and no
+// clearing; no other stream of the wavefront writes those cells meanwhile
+// and the other wavefronts own other rows, so the load-store pair is
+// race-free.
+template <int W, int NW, int G, int S, bool TAB, bool BC = false, bool RS = false>
 __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restrict__ fin,
                                                          float2 *__restrict__ g,
                                                          const int *__restrict__ perm,
@@ -96,6 +101,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
   constexpr int NPASS = G / NP;
   static_assert(G % NP == 0 && 64 % G == 0, "groups tile the 64-node chunk");
   static_assert(!(BC && TAB) && (!BC || RW <= 16), "one row-weight path; RW rows fit a DPP row");
+  static_assert(!RS || S == 1, "run load-store needs one stream per wavefront");
   // right-hand side blockIdx.y of a batched launch
   fin += (long)blockIdx.y * gg.bs_f;
   g += (long)blockIdx.y * gg.bs_g;
@@ -192,6 +198,35 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
 #pragma unroll
       for (int j = 0; j < RW; j++) acc[st][j][sl][0] = acc[st][j][sl][1] = float2{0.f, 0.f};
     }
+  };
+  // RS: the run's sums are the tile's values; store them, and load the next
+  // run's (after the store in program order: the runs' cells overlap)
+  auto run_cells = [&](int st, int sl, int j) {
+    return tile + (cur[st] & 0xfffff) + ((cur[st] >> 22) + j * NW) * plane + off[sl];
+  };
+  auto store = [&](int st) {
+#pragma unroll
+    for (int sl = 0; sl < SLOTS; sl++)
+      if (pv[sl]) {
+#pragma unroll
+        for (int j = 0; j < RW; j++) {
+          float2 *t = run_cells(st, sl, j);
+          t[0] = acc[st][j][sl][0];
+          t[1] = acc[st][j][sl][1];
+        }
+      }
+  };
+  auto load = [&](int st) {
+#pragma unroll
+    for (int sl = 0; sl < SLOTS; sl++)
+      if (pv[sl]) {
+#pragma unroll
+        for (int j = 0; j < RW; j++) {
+          const float2 *t = run_cells(st, sl, j);
+          acc[st][j][sl][0] = t[0];
+          acc[st][j][sl][1] = t[1];
+        }
+      }
   };
 
   // chunk of 64 nodes per stream, one per lane: packed corner, frac - 1/2,
@@ -329,8 +364,14 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
               a1[sl] = w1[st][k][sl];
             }
             if (key != cur[st]) {
-              flush(st);
-              cur[st] = key;
+              if constexpr (RS) {
+                store(st);
+                cur[st] = key;
+                load(st);
+              } else {
+                flush(st);
+                cur[st] = key;
+              }
             }
             rows_bcast<0, RW, SLOTS>(acc[st], brr[st][k], bri[st][k], a0, a1);
             continue;
@@ -378,7 +419,10 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
     }
   }
 #pragma unroll
-  for (int st = 0; st < S; st++) flush(st);
+  for (int st = 0; st < S; st++) {
+    if constexpr (RS) store(st);
+    else flush(st);
+  }
   __syncthreads();
   flush_tile<float, 3>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, sp.w, threadIdx.x,
                        blockDim.x);
