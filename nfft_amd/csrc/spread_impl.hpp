@@ -354,14 +354,18 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
 #pragma unroll
           for (int i1 = 0; i1 < W; i1++) {
             const cplx<R> *row = base + (i0 * E1 + i1) * E2;
+            // the row's taps are all requested before the first is used, so
+            // a wavefront keeps several LDS reads in flight
+            cplx<R> v[W];
+#pragma unroll
+            for (int w = 0; w < W; w++) v[w] = row[w];
             R sar = 0, sai = 0, sbr = 0, sbi = 0;
 #pragma unroll
             for (int w = 0; w < W; w++) {
-              const cplx<R> v = row[w];
-              sar += pa_[2][w] * v.x;
-              sai += pa_[2][w] * v.y;
-              sbr += pb_[2][w] * v.x;
-              sbi += pb_[2][w] * v.y;
+              sar += pa_[2][w] * v[w].x;
+              sai += pa_[2][w] * v[w].y;
+              sbr += pb_[2][w] * v[w].x;
+              sbi += pb_[2][w] * v[w].y;
             }
             qar += pa_[1][i1] * sar;
             qai += pa_[1][i1] * sai;
