@@ -355,17 +355,21 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
           for (int i1 = 0; i1 < W; i1++) {
             const cplx<R> *row = base + (i0 * E1 + i1) * E2;
             // the row's taps are all requested before the first is used, so
-            // a wavefront keeps several LDS reads in flight
+            // a wavefront keeps several LDS reads in flight (short windows
+            // read in the loop: the compiler would hoist every row)
             cplx<R> v[W];
+            if constexpr (W >= 8) {
 #pragma unroll
-            for (int w = 0; w < W; w++) v[w] = row[w];
+              for (int w = 0; w < W; w++) v[w] = row[w];
+            }
             R sar = 0, sai = 0, sbr = 0, sbi = 0;
 #pragma unroll
             for (int w = 0; w < W; w++) {
-              sar += pa_[2][w] * v[w].x;
-              sai += pa_[2][w] * v[w].y;
-              sbr += pb_[2][w] * v[w].x;
-              sbi += pb_[2][w] * v[w].y;
+              const cplx<R> t = W >= 8 ? v[w] : row[w];
+              sar += pa_[2][w] * t.x;
+              sai += pa_[2][w] * t.y;
+              sbr += pb_[2][w] * t.x;
+              sbi += pb_[2][w] * t.y;
             }
             qar += pa_[1][i1] * sar;
             qai += pa_[1][i1] * sai;
@@ -403,12 +407,18 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
 #pragma unroll
       for (int i0 = 0; i0 < W; i0++) {
         const cplx<R> *row = base + i0 * E1;
+        // the row's taps are all requested before the first is used
+        cplx<R> v[W];
+        if constexpr (W >= 8) {
+#pragma unroll
+          for (int w = 0; w < W; w++) v[w] = row[w];
+        }
         R sr = 0, si = 0;
 #pragma unroll
         for (int w = 0; w < W; w++) {
-          const cplx<R> v = row[w];
-          sr += psi[1][w] * v.x;
-          si += psi[1][w] * v.y;
+          const cplx<R> t = W >= 8 ? v[w] : row[w];
+          sr += psi[1][w] * t.x;
+          si += psi[1][w] * t.y;
         }
         ar += psi[0][i0] * sr;
         ai += psi[0][i0] * si;
@@ -420,12 +430,17 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
 #pragma unroll
         for (int i1 = 0; i1 < W; i1++) {
           const cplx<R> *row = base + (i0 * E1 + i1) * E2;
+          cplx<R> v[W];
+          if constexpr (W >= 8) {
+#pragma unroll
+            for (int w = 0; w < W; w++) v[w] = row[w];
+          }
           R sr = 0, si = 0;
 #pragma unroll
           for (int w = 0; w < W; w++) {
-            const cplx<R> v = row[w];
-            sr += psi[2][w] * v.x;
-            si += psi[2][w] * v.y;
+            const cplx<R> t = W >= 8 ? v[w] : row[w];
+            sr += psi[2][w] * t.x;
+            si += psi[2][w] * t.y;
           }
           qr += psi[1][i1] * sr;
           qi += psi[1][i1] * si;
