@@ -47,6 +47,10 @@ struct BinGeom {
   // the plan's interpolation kernel does not use them
   const int2 *slots;
   const int2 *islots;
+  // LDS row pitch (cells) of the 3-D lane-per-node interpolation tile:
+  // E[2], or padded so that lanes reading windows of different cells of one
+  // plane spread over more LDS banks
+  int ip2;
 };
 
 // Kaiser-Bessel window of one dimension, scaled by 1/phi(0) so that the B

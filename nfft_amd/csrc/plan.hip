@@ -220,6 +220,18 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     geom.nb[t] = t < d ? (int)ceil_div(n[t], bs[t]) : 1;
     geom.E[t] = t < d ? bs[t] + 2 * m + 1 : 1;
   }
+  geom.ip2 = geom.E[2];
+  if (d == 3) {
+    // NFFT_HIP_IPITCH=1: interpolation tile rows padded to 4 (mod 8) cells
+    // (an 8 x 8 plane of distinct cells then maps to 32 banks of 8 bytes
+    // 2-way instead of 4-way at 17 cells), when the tile still fits
+    const char *ip = std::getenv("NFFT_HIP_IPITCH");
+    if (ip && std::strcmp(ip, "1") == 0) {
+      int p2 = geom.E[2];
+      while (p2 % 8 != 4) p2++;
+      if ((size_t)geom.E[0] * geom.E[1] * p2 * sizeof(cplx<R>) <= 64 * 1024) geom.ip2 = p2;
+    }
+  }
   // nodes per work item; larger 3-D spreading items write fewer tiles for
   // the reduction (measured on the 3-D kooshball: 2048 -> 4096 halves the
   // tile reduction at equal spreading time, larger items end the launch late)

@@ -232,14 +232,14 @@ __device__ __forceinline__ void node_window_sym(const int *__restrict__ cell,
 }
 
 // Stages the grid region of a 3-D work item (E0 x E1 x E2 cells from
-// lo - m, periodic) into LDS.  Each thread keeps one position along the last
+// lo - m, periodic) into LDS, rows gg.ip2 cells apart.  Each thread keeps one position along the last
 // dimension and walks whole rows, so per element it does one table lookup
 // per leading dimension instead of divisions and modulos.
 template <typename R>
 __device__ __forceinline__ void load_region_3d(const cplx<R> *__restrict__ g, cplx<R> *tile,
                                                const int (&lo)[3], const BinGeom &gg,
                                                long *rowoff0, long *rowoff1) {
-  const int E0 = gg.E[0], E1 = gg.E[1], E2 = gg.E[2];
+  const int E0 = gg.E[0], E1 = gg.E[1], E2 = gg.E[2], P2 = gg.ip2;
   auto wrap = [](int v, int n) {
     v %= n;
     return v < 0 ? v + n : v;
@@ -264,7 +264,7 @@ __device__ __forceinline__ void load_region_3d(const cplx<R> *__restrict__ g, cp
       for (int u = 0; u < B; u++) {
         dst[u] = -1;
         if (e0 < E0) {
-          dst[u] = (e0 * E1 + e1) * E2 + e2;
+          dst[u] = (e0 * E1 + e1) * P2 + e2;
           v[u] = g[rowoff0[e0] + rowoff1[e1] + col];
           e1 += rows_per_pass;
           while (e1 >= E1) {
@@ -309,7 +309,8 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
     for (int q = threadIdx.x; q < tot; q += blockDim.x) tile[q] = g[region_to_grid<D>(q, lo, gg)];
   }
   const int E1 = D >= 2 ? gg.E[1] : 1;
-  const int E2 = D == 3 ? gg.E[2] : 1;
+  // the tile's row pitch (load_region_3d)
+  const int E2 = D == 3 ? gg.ip2 : 1;
   __syncthreads();
   if constexpr (PAIR && (D == 2 || D == 3)) {
     const int2 is = gg.islots[blockIdx.x];
@@ -1542,7 +1543,9 @@ template <typename R>
 typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeom &gg,
                                           bool spread, int variant, size_t &lds, int &threads,
                                           bool shared_poly = false) {
-  const size_t bytes = tile_bytes<R>(gg);
+  size_t bytes = tile_bytes<R>(gg);
+  // the lane-per-node 3-D interpolation tile has rows gg.ip2 cells apart
+  if (!spread && variant == 0 && gg.d == 3) bytes = bytes / gg.E[2] * gg.ip2;
   const bool tile = bytes <= kTileBudget;
   lds = tile ? bytes : 0;
   threads = 256;
