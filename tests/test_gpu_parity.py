@@ -280,10 +280,14 @@ def test_four_dimensional(precision, m, N, n):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("pairs", ["0", "1"])
-def test_interp_pairs_on_and_off(pairs, monkeypatch):
+@pytest.mark.parametrize("pitch", ["0", "1"])
+def test_interp_pairs_on_and_off(pairs, pitch, monkeypatch):
     """3-D FP32 interpolation with and without same-cell node pairs (dense
-    kooshball centre: most nodes share their cell) against the oracle."""
+    kooshball centre: most nodes share their cell) against the oracle, with
+    the interpolation tile's rows at their natural length and padded
+    (NFFT_HIP_IPITCH)."""
     monkeypatch.setenv("NFFT_HIP_PAIRS", pairs)
+    monkeypatch.setenv("NFFT_HIP_IPITCH", pitch)
     N = [16, 16, 16]
     x = W.kooshball(96, 64, seed=5)
     fh = W.complex_normal(int(np.prod(N)), seed=81)
