@@ -180,6 +180,26 @@ def test_empty_and_edge_sizes():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("variant,d", [("runs2sbc", 3), ("runs2rs", 3), ("runs2d", 2)])
+def test_spreading_runs_edge_cases(variant, d, monkeypatch):
+    """FP32 run-merging B^T kernels on work items that stress their node
+    loops: 9001 nodes in one cell (runs across many 64-node chunks and
+    several work items, streams of unequal length, partial last chunks), a
+    lone node in a bin of its own (an item of one node: a stream without
+    nodes), nodes on the -0.5 edge, and uniform nodes in one corner."""
+    monkeypatch.setenv("NFFT_HIP_SPREAD", variant)
+    rng = np.random.default_rng(51)
+    N = [16] * d
+    x = np.concatenate([np.zeros((9001, d)), np.full((7, d), -0.5),
+                        rng.uniform(-0.5, -0.1, (513, d)), np.full((1, d), 0.37)])
+    fh = W.complex_normal(int(np.prod(N)), seed=52)
+    f = W.complex_normal(x.shape[0], seed=53)
+    (fg, fr, _), (hg, hr, _) = run_pair(N, x, fh, f, 4, "single")
+    print("%s %dD edge cases: adjoint rel l2 %.2e, trafo %.2e" % (variant, d, rel_errs(hg, hr)[0], rel_errs(fg, fr)[0]))
+    assert rel_errs(hg, hr)[0] < 1e-5 and rel_errs(fg, fr)[0] < 1e-5
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["double", "single"])
 def test_non_power_of_two_grid(precision):
     # oversampled lengths with factors 3, 5 exercise the mixed-radix FFT
