@@ -222,14 +222,16 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   }
   geom.ip2 = geom.E[2];
   if (d == 3) {
-    // NFFT_HIP_IPITCH=1: interpolation tile rows padded to 4 (mod 8) cells
-    // (an 8 x 8 plane of distinct cells then maps to 32 banks of 8 bytes
-    // 2-way instead of 4-way at 17 cells), when the tile still fits
+    // interpolation tile rows padded to 4 (mod 8) cells (an 8 x 8 plane of
+    // distinct cells then maps to 32 banks of 8 bytes 2-way instead of 4-way
+    // at 17 cells), when the tile still fits; NFFT_HIP_IPITCH=0: unpadded
     const char *ip = std::getenv("NFFT_HIP_IPITCH");
-    if (ip && std::strcmp(ip, "1") == 0) {
+    if (!(ip && std::strcmp(ip, "0") == 0)) {
+      // (at most 3 cells of padding per row: measured at m = 4, 17 -> 20)
       int p2 = geom.E[2];
       while (p2 % 8 != 4) p2++;
-      if ((size_t)geom.E[0] * geom.E[1] * p2 * sizeof(cplx<R>) <= 64 * 1024) geom.ip2 = p2;
+      if (p2 - geom.E[2] <= 3 && (size_t)geom.E[0] * geom.E[1] * p2 * sizeof(cplx<R>) <= 64 * 1024)
+        geom.ip2 = p2;
     }
   }
   // nodes per work item; larger 3-D spreading items write fewer tiles for
