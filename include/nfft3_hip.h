@@ -81,6 +81,12 @@ typedef struct nfft_hip_graph_s nfft_hip_graph;
 NFFT_HIP_DECLARE_EXT(nfftf_, nfftf_plan, float)
 NFFT_HIP_DECLARE_EXT(nfft_, nfft_plan, double)
 
+/* Diagnostics: the split of one bin of cnt nodes into B-stage work items,
+ * as the node binning computes it (at most max_per_sub nodes per item, at
+ * most max_items items, 0: no limit).  Returns the item count k and writes
+ * first[0..k-1] (offset in the bin) and count[0..k-1]; host only. */
+int nfft_hip_work_split(int cnt, int max_per_sub, int max_items, int *first, int *count);
+
 /* Replay a captured graph (enqueued, no wait), wait for it, release it. */
 void nfft_hip_graph_launch(nfft_hip_graph *graph);
 void nfft_hip_graph_synchronize(nfft_hip_graph *graph);

@@ -40,6 +40,22 @@ template <> struct family<double> {
 
 inline long ceil_div(long a, long b) { return (a + b - 1) / b; }
 
+// Work items of a bin of cnt nodes: ceil(cnt / max_per_sub) of them, at most
+// max_items (0: no limit; dense bins then get longer items and fewer tiles).
+__host__ __device__ inline int items_of(int cnt, int max_per_sub, int max_items) {
+  const int k = (cnt + max_per_sub - 1) / max_per_sub;
+  return max_items > 0 ? (k < max_items ? k : max_items) : k;
+}
+
+// Item s of k over a bin of cnt nodes: nodes [first, first + count) of the
+// bin, split evenly (counts differ by at most one, every count >= 1 when
+// k <= cnt, and they sum to cnt).
+__host__ __device__ inline void split_item(int cnt, int k, int s, int &first, int &count) {
+  const long lo = (long)s * cnt / k, hi = (long)(s + 1) * cnt / k;
+  first = (int)lo;
+  count = (int)(hi - lo);
+}
+
 // The HIP runtime's per-thread last-error state also records failures of
 // calls made elsewhere in the process (other libraries, the runtime's own
 // fork handling).  Every entry point clears it first, so that its

@@ -157,14 +157,6 @@ __global__ void k_node_place(BinGeom gg, const int *__restrict__ cell_tmp,
   }
 }
 
-// Work items of a bin: ceil(count / max_per_sub) of them, at most max_items
-// (0: no limit; dense bins then get longer items and fewer tiles), the
-// bin's nodes split evenly
-__device__ __forceinline__ int items_of(int cnt, int max_per_sub, int max_items) {
-  const int k = (cnt + max_per_sub - 1) / max_per_sub;
-  return max_items > 0 ? min(k, max_items) : k;
-}
-
 __global__ void k_count_subs(const int *__restrict__ hist, long nbins, int max_per_sub,
                              int max_items, int *__restrict__ nsub_of) {
   long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -180,12 +172,11 @@ __global__ void k_fill_subs(const int *__restrict__ hist, const int *__restrict_
   int cnt = hist[b];
   int s0 = sub_start[b];
   const int k = items_of(cnt, max_per_sub, max_items);
-  const int per = k > 0 ? (cnt + k - 1) / k : 0;
   for (int s = 0; s < k; s++) {
-    int first = bin_start[b] + s * per;
-    int c = min(per, cnt - s * per);
+    int first, c;
+    split_item(cnt, k, s, first, c);
     // .w: the item's place in bin order (its spreading tile's slot)
-    subs[s0 + s] = make_int4((int)b, first, c, s0 + s);
+    subs[s0 + s] = make_int4((int)b, bin_start[b] + first, c, s0 + s);
   }
 }
 
@@ -503,3 +494,13 @@ template void bin_nodes<float>(Engine<float> &);
 template void bin_nodes<double>(Engine<double> &);
 
 }  // namespace nfft_hip
+
+// Diagnostics hook (host only): the work-item split of one bin as the
+// binning kernels compute it.  Returns the number of items k and writes
+// each item's first node (relative to the bin) and node count.
+extern "C" int nfft_hip_work_split(int cnt, int max_per_sub, int max_items, int *first,
+                                   int *count) {
+  const int k = nfft_hip::items_of(cnt, max_per_sub, max_items);
+  for (int s = 0; s < k; s++) nfft_hip::split_item(cnt, k, s, first[s], count[s]);
+  return k;
+}

@@ -1069,7 +1069,7 @@ struct PlanApi {
     clear_launch_error();
     Engine<R> *e = eng(p);
     if (e->direct_only) {
-      if (!e->nodes_ready) precompute(p);
+      e->upload_nodes(p->x);  // the NDFT reads the current x (nfft.c:145-297)
       e->trafo_direct((const cplx<R> *)fh, (cplx<R> *)f);
       return;
     }
@@ -1082,7 +1082,7 @@ struct PlanApi {
     clear_launch_error();
     Engine<R> *e = eng(p);
     if (e->direct_only) {
-      if (!e->nodes_ready) precompute(p);
+      e->upload_nodes(p->x);  // the NDFT reads the current x (nfft.c:145-297)
       e->adjoint_direct((const cplx<R> *)f, (cplx<R> *)fh);
       return;
     }
@@ -1104,7 +1104,7 @@ struct PlanApi {
     clear_launch_error();
     Engine<R> *e = eng(p);
     if (e->direct_only) {
-      if (!e->nodes_ready) precompute(p);
+      e->upload_nodes(p->x);  // the NDFT reads the current x (nfft.c:145-297)
     } else {
       ensure_nodes(p, e);
     }
@@ -1132,7 +1132,7 @@ struct PlanApi {
       fw[j] = cplx<R>{wj * f[2 * j], wj * f[2 * j + 1]};
     }
     if (e->direct_only) {
-      if (!e->nodes_ready) precompute(p);
+      e->upload_nodes(p->x);  // the NDFT reads the current x (nfft.c:145-297)
     } else {
       ensure_nodes(p, e);
     }
@@ -1188,7 +1188,7 @@ struct PlanApi {
       if (e->timing || e->deferred)
         family<R>::fatal("nfft: switch stage timing off before capturing a graph");
       if (e->direct_only) {
-        if (!e->nodes_ready) precompute(plans[i]);
+        e->upload_nodes(plans[i]->x);
       } else {
         ensure_nodes(plans[i], e);
         e->reserve_batch(nrhs);  // batch buffers exist before the capture

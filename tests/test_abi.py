@@ -111,3 +111,26 @@ def test_no_device_fails_loudly(libpath):
     assert r.returncode != 0
     assert "not reached" not in r.stdout
     assert "no HIP device" in (r.stderr + r.stdout)
+
+
+@pytest.mark.parametrize("cnt", [1, 2, 7, 255, 256, 257, 4095, 4096, 4097, 76825, 1 << 20])
+@pytest.mark.parametrize("max_per_sub,cap", [(64, 0), (256, 0), (256, 300), (256, 3), (1024, 0),
+                                             (1024, 1), (4096, 2), (512, 100000)])
+def test_work_split_covers_bin(libpath, cnt, max_per_sub, cap):
+    """The B-stage work items of one bin cover its nodes exactly once, each
+    item non-empty and within the size limit unless the item cap forces longer
+    items (the node binning's split, host copy of the device function)."""
+    L = ctypes.CDLL(libpath)
+    k_max = (cnt + max_per_sub - 1) // max_per_sub
+    first = (ctypes.c_int * max(1, k_max))()
+    count = (ctypes.c_int * max(1, k_max))()
+    k = L.nfft_hip_work_split(cnt, max_per_sub, cap, first, count)
+    assert k == (min(k_max, cap) if cap > 0 else k_max)
+    f = list(first[:k])
+    c = list(count[:k])
+    assert all(x >= 1 for x in c)
+    assert sum(c) == cnt
+    assert f[0] == 0 and all(f[i] + c[i] == f[i + 1] for i in range(k - 1))
+    assert max(c) - min(c) <= 1
+    if cap == 0 or k_max <= cap:
+        assert max(c) <= max_per_sub

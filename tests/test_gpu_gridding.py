@@ -113,3 +113,35 @@ def test_gridding_with_density_weights_beats_unweighted():
     if "learned" in errs:
         assert errs["learned"] < 0.6 * errs["plain"]
         assert errs["learned"] < 1.15 * errs["pipe_menon"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_direct_only_plan_reads_current_nodes(precision):
+    """A plan too small for the fast path (N_t <= m) runs the NDFT, which
+    reads the caller's x on every call (reference nfft.c:145-297 reads
+    ths->x each time): changing x between two gridding calls, host and device
+    forms, must change the result accordingly."""
+    N = [4, 4]
+    M = 40
+    p = nf.Plan(N, M, n=[8, 8], m=4, flags=FLAGS, precision=precision)
+    tol = 1e-12 if precision == "double" else 1e-5
+    try:
+        f = W.complex_normal(M, seed=61).astype(p.cdtype)
+        w = (0.5 + np.random.default_rng(62).random(M)).astype(p.rdtype)
+        cd = torch.complex128 if precision == "double" else torch.complex64
+        d_f = torch.from_numpy(f).to("cuda")
+        d_w = torch.from_numpy(w).to("cuda")
+        d_fh = torch.empty(p.p.N_total, dtype=cd, device="cuda")
+        for seed in (63, 64):
+            p.x[:] = W.uniform(M, 2, seed=seed).astype(p.rdtype)
+            xr = p.x.astype(np.float64)
+            ref = O.ndft_adjoint(N, xr, f.astype(np.complex128) * w.astype(np.float64))
+            p.f[:] = f
+            p.gridding(w)
+            assert rel(p.f_hat.astype(np.complex128), ref) <= tol
+            p.gridding_device(d_f.data_ptr(), d_w.data_ptr(), d_fh.data_ptr())
+            p.synchronize()
+            assert rel(d_fh.cpu().numpy().astype(np.complex128), ref) <= tol
+    finally:
+        p.finalize()
