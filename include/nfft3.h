@@ -208,6 +208,24 @@ typedef struct { NFFT_HIP_SOLVER_FIELDS(double, fftw_complex, nfft_mv_plan_compl
 NFFT_HIP_DECLARE_SOLVER(solverf_, solverf_plan_complex, nfftf_mv_plan_complex)
 NFFT_HIP_DECLARE_SOLVER(solver_, solver_plan_complex, nfft_mv_plan_complex)
 
+/* Real-valued solver on an mv plan with real f_hat and f (reference
+ * include/nfft3.h:786-819, kernel/solver/solver.c:392-714): the same four
+ * methods, flags and weights; y, the iterate and every work vector hold one
+ * real per entry.  Runs on the host around the caller's mv_trafo /
+ * mv_adjoint, with the reference's pointer swaps. */
+typedef struct { NFFT_HIP_SOLVER_FIELDS(float, float, nfftf_mv_plan_double) } solverf_plan_double;
+typedef struct { NFFT_HIP_SOLVER_FIELDS(double, double, nfft_mv_plan_double) } solver_plan_double;
+
+#define NFFT_HIP_DECLARE_SOLVER_DOUBLE(S, PLAN, MV)                          \
+  void S##init_advanced_double(PLAN *ths, MV *mv, unsigned flags);           \
+  void S##init_double(PLAN *ths, MV *mv);                                    \
+  void S##before_loop_double(PLAN *ths);                                     \
+  void S##loop_one_step_double(PLAN *ths);                                   \
+  void S##finalize_double(PLAN *ths);
+
+NFFT_HIP_DECLARE_SOLVER_DOUBLE(solverf_, solverf_plan_double, nfftf_mv_plan_double)
+NFFT_HIP_DECLARE_SOLVER_DOUBLE(solver_, solver_plan_double, nfft_mv_plan_double)
+
 /* ----------------------------------------------------------------------- */
 /* Utilities used by callers of the path (reference include/nfft3.h:839-894) */
 /* ----------------------------------------------------------------------- */
@@ -238,6 +256,24 @@ NFFT_HIP_DECLARE_SOLVER(solver_, solver_plan_complex, nfft_mv_plan_complex)
 
 NFFT_HIP_DECLARE_UTIL(nfftf_, float, fftwf_complex)
 NFFT_HIP_DECLARE_UTIL(nfft_, double, fftw_complex)
+
+/* ----------------------------------------------------------------------- */
+/* Damping factors for the solver's w_hat (PRECOMPUTE_DAMP): the library's  */
+/* exported helpers of reference include/infft.h:1506-1520 (bodies          */
+/* kernel/util/damp.c:25-142).  Factor kk is the sum of a kernel's Fourier  */
+/* coefficients at kk and kk + 1: Fejer 2/N^2 (1 - |2kk+1|/N), Jackson      */
+/* (power 4, n = (N/2+1)/2), generalised Jackson (power 8, n = (N/2+3)/4),  */
+/* Sobolev |k|^(-2 mu), multiquadric (k^2 + c^2)^(-mu).                     */
+/* ----------------------------------------------------------------------- */
+#define NFFT_HIP_DECLARE_DAMP(P, R)                                          \
+  R P##modified_fejer(const NFFT_INT N, const NFFT_INT kk);                  \
+  R P##modified_jackson2(const NFFT_INT N, const NFFT_INT kk);               \
+  R P##modified_jackson4(const NFFT_INT N, const NFFT_INT kk);               \
+  R P##modified_sobolev(const R mu, const NFFT_INT kk);                      \
+  R P##modified_multiquadric(const R mu, const R c, const NFFT_INT kk);
+
+NFFT_HIP_DECLARE_DAMP(nfftf_, float)
+NFFT_HIP_DECLARE_DAMP(nfft_, double)
 
 #ifdef __cplusplus
 } /* extern "C" */

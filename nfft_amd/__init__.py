@@ -97,6 +97,30 @@ class solver_plan_complex(ctypes.Structure):
     _fields_ = _solver_fields(ctypes.c_double)
 
 
+# real-valued solver (solver[f]_plan_double) and its mv plan prefix
+# (reference include/nfft3.h:54-60, 786-819): one real per vector entry
+class solverf_plan_double(ctypes.Structure):
+    _fields_ = _solver_fields(ctypes.c_float)
+
+
+class solver_plan_double(ctypes.Structure):
+    _fields_ = _solver_fields(ctypes.c_double)
+
+
+def _mv_fields(R):
+    P = ctypes.POINTER
+    return [("N_total", _INT), ("M_total", _INT), ("f_hat", P(R)), ("f", P(R)),
+            ("mv_trafo", _FN), ("mv_adjoint", _FN)]
+
+
+class nfftf_mv_plan_double(ctypes.Structure):
+    _fields_ = _mv_fields(ctypes.c_float)
+
+
+class nfft_mv_plan_double(ctypes.Structure):
+    _fields_ = _mv_fields(ctypes.c_double)
+
+
 _LIB = None
 
 

@@ -197,44 +197,61 @@ DevCgnr<R> *find_dev(void *ip) {
 }
 
 // ------------------------------------------------------------------ host BLAS-1
-template <typename R>
-static double hdot(const R *x, const R *w, long n) {
+// K reals per vector entry: 2 for complex vectors, 1 for real ones
+template <typename R, int K>
+static double hdot(const R *x, const R *w, long n) {  // sum_k w_k |x_k|^2
   double s = 0;
   for (long k = 0; k < n; k++) {
-    const double a = (double)x[2 * k] * x[2 * k] + (double)x[2 * k + 1] * x[2 * k + 1];
+    double a = 0;
+    for (int c = 0; c < K; c++) a += (double)x[K * k + c] * x[K * k + c];
     s += w ? (double)w[k] * a : a;
   }
   return s;
 }
-template <typename R>
+template <typename R, int K>
 static void hcp_w(R *x, const R *w, const R *y, long n) {  // x = w y
   for (long k = 0; k < n; k++) {
     const R s = w ? w[k] : R(1);
-    x[2 * k] = s * y[2 * k];
-    x[2 * k + 1] = s * y[2 * k + 1];
+    for (int c = 0; c < K; c++) x[K * k + c] = s * y[K * k + c];
   }
 }
-template <typename R>
+template <typename R, int K>
 static void haxpy(R *x, R a, const R *y, long n) {  // x = a x + y
-  for (long k = 0; k < 2 * n; k++) x[k] = a * x[k] + y[k];
+  for (long k = 0; k < K * n; k++) x[k] = a * x[k] + y[k];
 }
-template <typename R>
+template <typename R, int K>
 static void hxpawy(R *x, R a, const R *w, const R *y, long n) {  // x += a w y
   for (long k = 0; k < n; k++) {
     const R s = w ? a * w[k] : a;
-    x[2 * k] += s * y[2 * k];
-    x[2 * k + 1] += s * y[2 * k + 1];
+    for (int c = 0; c < K; c++) x[K * k + c] += s * y[K * k + c];
   }
 }
 
 // ------------------------------------------------------------------ solver API
 template <typename R, typename SPLAN, typename MV, typename C>
 struct SolverApi {
+  // reals per vector entry (complex solver 2, real solver 1)
+  static constexpr int K = (int)(sizeof(C) / sizeof(R));
+
+  // Vectors of the complex solver are page-locked (the device CGNR copies
+  // them every step); the real solver's are plain nfft[f]_malloc memory, as
+  // in the reference (solver.c:392-428), so it needs no GPU at all.
+  static void *valloc(size_t bytes) {
+    if constexpr (K == 2) return host_array_alloc<R>(bytes);
+    else if constexpr (sizeof(R) == 4) return nfftf_malloc(bytes);
+    else return nfft_malloc(bytes);
+  }
+  static void vfree(void *p) {
+    if constexpr (K == 2) host_array_free<R>(p);
+    else if constexpr (sizeof(R) == 4) nfftf_free(p);
+    else nfft_free(p);
+  }
+
   static void init_advanced(SPLAN *ths, MV *mv, unsigned flags) {
     ths->mv = mv;
     ths->flags = flags;
     const long M = (long)mv->M_total, N = (long)mv->N_total;
-    auto cvec = [&](long n) { return (C *)host_array_alloc<R>(sizeof(C) * (size_t)n); };
+    auto cvec = [&](long n) { return (C *)valloc(sizeof(C) * (size_t)n); };
     ths->y = cvec(M);
     ths->r_iter = cvec(M);
     ths->f_hat_iter = cvec(N);
@@ -253,14 +270,16 @@ struct SolverApi {
       ths->v_iter = cvec(M);
     }
     if (flags & CGNE) ths->z_hat_iter = ths->p_hat_iter;
-    if (flags & PRECOMPUTE_WEIGHT) ths->w = (R *)host_array_alloc<R>(sizeof(R) * (size_t)M);
-    if (flags & PRECOMPUTE_DAMP) ths->w_hat = (R *)host_array_alloc<R>(sizeof(R) * (size_t)N);
+    if (flags & PRECOMPUTE_WEIGHT) ths->w = (R *)valloc(sizeof(R) * (size_t)M);
+    if (flags & PRECOMPUTE_DAMP) ths->w_hat = (R *)valloc(sizeof(R) * (size_t)N);
     ths->alpha_iter = ths->beta_iter = 0;
     ths->dot_r_iter = ths->dot_r_iter_old = 0;
     ths->dot_z_hat_iter = ths->dot_z_hat_iter_old = 0;
     ths->dot_p_hat_iter = ths->dot_v_iter = 0;
 
-    Engine<R> *e = engine_of_mv<R>(mv);
+    // the real-valued solver always runs on the host: its mv plan is the
+    // caller's own (this library's plans are complex)
+    Engine<R> *e = K == 2 ? engine_of_mv<R>(mv) : nullptr;
     if ((flags & CGNR) && e && !e->direct_only) {
       DevCgnr<R> *dv = new DevCgnr<R>();
       dv->alloc(e, M, N, flags & PRECOMPUTE_WEIGHT, flags & PRECOMPUTE_DAMP);
@@ -375,15 +394,15 @@ struct SolverApi {
     NFFT_HIP_CSWAP(ths->r_iter, mv->f);
     mv->mv_trafo(mv);
     NFFT_HIP_CSWAP(ths->r_iter, mv->f);
-    haxpy<R>((R *)ths->r_iter, R(-1), (const R *)ths->y, M);
+    haxpy<R, K>((R *)ths->r_iter, R(-1), (const R *)ths->y, M);
     if (!(ths->flags & LANDWEBER) || (ths->flags & NORMS_FOR_LANDWEBER))
-      ths->dot_r_iter = (R)hdot<R>((const R *)ths->r_iter, ths->flags & PRECOMPUTE_WEIGHT ? ths->w : nullptr, M);
-    hcp_w<R>((R *)mv->f, ths->flags & PRECOMPUTE_WEIGHT ? ths->w : nullptr, (const R *)ths->r_iter, M);
+      ths->dot_r_iter = (R)hdot<R, K>((const R *)ths->r_iter, ths->flags & PRECOMPUTE_WEIGHT ? ths->w : nullptr, M);
+    hcp_w<R, K>((R *)mv->f, ths->flags & PRECOMPUTE_WEIGHT ? ths->w : nullptr, (const R *)ths->r_iter, M);
     NFFT_HIP_CSWAP(ths->z_hat_iter, mv->f_hat);
     mv->mv_adjoint(mv);
     NFFT_HIP_CSWAP(ths->z_hat_iter, mv->f_hat);
     if (!(ths->flags & LANDWEBER) || (ths->flags & NORMS_FOR_LANDWEBER))
-      ths->dot_z_hat_iter = (R)hdot<R>((const R *)ths->z_hat_iter, ths->flags & PRECOMPUTE_DAMP ? ths->w_hat : nullptr, N);
+      ths->dot_z_hat_iter = (R)hdot<R, K>((const R *)ths->z_hat_iter, ths->flags & PRECOMPUTE_DAMP ? ths->w_hat : nullptr, N);
     if (ths->flags & CGNE) ths->dot_p_hat_iter = ths->dot_z_hat_iter;
     if (ths->flags & CGNR) std::memcpy(ths->p_hat_iter, ths->z_hat_iter, sizeof(C) * (size_t)N);
   }
@@ -393,19 +412,19 @@ struct SolverApi {
     const long M = (long)mv->M_total, N = (long)mv->N_total;
     const R *wh = ths->flags & PRECOMPUTE_DAMP ? ths->w_hat : nullptr;
     const R *w = ths->flags & PRECOMPUTE_WEIGHT ? ths->w : nullptr;
-    hxpawy<R>((R *)ths->f_hat_iter, ths->alpha_iter, wh, (const R *)ths->z_hat_iter, N);
+    hxpawy<R, K>((R *)ths->f_hat_iter, ths->alpha_iter, wh, (const R *)ths->z_hat_iter, N);
     std::memcpy(mv->f_hat, ths->f_hat_iter, sizeof(C) * (size_t)N);
     NFFT_HIP_CSWAP(ths->r_iter, mv->f);
     mv->mv_trafo(mv);
     NFFT_HIP_CSWAP(ths->r_iter, mv->f);
-    haxpy<R>((R *)ths->r_iter, R(-1), (const R *)ths->y, M);
-    if (ths->flags & NORMS_FOR_LANDWEBER) ths->dot_r_iter = (R)hdot<R>((const R *)ths->r_iter, w, M);
-    hcp_w<R>((R *)mv->f, w, (const R *)ths->r_iter, M);
+    haxpy<R, K>((R *)ths->r_iter, R(-1), (const R *)ths->y, M);
+    if (ths->flags & NORMS_FOR_LANDWEBER) ths->dot_r_iter = (R)hdot<R, K>((const R *)ths->r_iter, w, M);
+    hcp_w<R, K>((R *)mv->f, w, (const R *)ths->r_iter, M);
     NFFT_HIP_CSWAP(ths->z_hat_iter, mv->f_hat);
     mv->mv_adjoint(mv);
     NFFT_HIP_CSWAP(ths->z_hat_iter, mv->f_hat);
     if (ths->flags & NORMS_FOR_LANDWEBER)
-      ths->dot_z_hat_iter = (R)hdot<R>((const R *)ths->z_hat_iter, wh, N);
+      ths->dot_z_hat_iter = (R)hdot<R, K>((const R *)ths->z_hat_iter, wh, N);
   }
 
   static void host_steepest(SPLAN *ths) {
@@ -413,20 +432,20 @@ struct SolverApi {
     const long M = (long)mv->M_total, N = (long)mv->N_total;
     const R *wh = ths->flags & PRECOMPUTE_DAMP ? ths->w_hat : nullptr;
     const R *w = ths->flags & PRECOMPUTE_WEIGHT ? ths->w : nullptr;
-    hcp_w<R>((R *)mv->f_hat, wh, (const R *)ths->z_hat_iter, N);
+    hcp_w<R, K>((R *)mv->f_hat, wh, (const R *)ths->z_hat_iter, N);
     NFFT_HIP_CSWAP(ths->v_iter, mv->f);
     mv->mv_trafo(mv);
     NFFT_HIP_CSWAP(ths->v_iter, mv->f);
-    ths->dot_v_iter = (R)hdot<R>((const R *)ths->v_iter, w, M);
+    ths->dot_v_iter = (R)hdot<R, K>((const R *)ths->v_iter, w, M);
     ths->alpha_iter = ths->dot_z_hat_iter / ths->dot_v_iter;
-    hxpawy<R>((R *)ths->f_hat_iter, ths->alpha_iter, wh, (const R *)ths->z_hat_iter, N);
-    hxpawy<R>((R *)ths->r_iter, -ths->alpha_iter, nullptr, (const R *)ths->v_iter, M);
-    ths->dot_r_iter = (R)hdot<R>((const R *)ths->r_iter, w, M);
-    hcp_w<R>((R *)mv->f, w, (const R *)ths->r_iter, M);
+    hxpawy<R, K>((R *)ths->f_hat_iter, ths->alpha_iter, wh, (const R *)ths->z_hat_iter, N);
+    hxpawy<R, K>((R *)ths->r_iter, -ths->alpha_iter, nullptr, (const R *)ths->v_iter, M);
+    ths->dot_r_iter = (R)hdot<R, K>((const R *)ths->r_iter, w, M);
+    hcp_w<R, K>((R *)mv->f, w, (const R *)ths->r_iter, M);
     NFFT_HIP_CSWAP(ths->z_hat_iter, mv->f_hat);
     mv->mv_adjoint(mv);
     NFFT_HIP_CSWAP(ths->z_hat_iter, mv->f_hat);
-    ths->dot_z_hat_iter = (R)hdot<R>((const R *)ths->z_hat_iter, wh, N);
+    ths->dot_z_hat_iter = (R)hdot<R, K>((const R *)ths->z_hat_iter, wh, N);
   }
 
   static void host_cgnr(SPLAN *ths) {
@@ -434,23 +453,23 @@ struct SolverApi {
     const long M = (long)mv->M_total, N = (long)mv->N_total;
     const R *wh = ths->flags & PRECOMPUTE_DAMP ? ths->w_hat : nullptr;
     const R *w = ths->flags & PRECOMPUTE_WEIGHT ? ths->w : nullptr;
-    hcp_w<R>((R *)mv->f_hat, wh, (const R *)ths->p_hat_iter, N);
+    hcp_w<R, K>((R *)mv->f_hat, wh, (const R *)ths->p_hat_iter, N);
     NFFT_HIP_CSWAP(ths->v_iter, mv->f);
     mv->mv_trafo(mv);
     NFFT_HIP_CSWAP(ths->v_iter, mv->f);
-    ths->dot_v_iter = (R)hdot<R>((const R *)ths->v_iter, w, M);
+    ths->dot_v_iter = (R)hdot<R, K>((const R *)ths->v_iter, w, M);
     ths->alpha_iter = ths->dot_z_hat_iter / ths->dot_v_iter;
-    hxpawy<R>((R *)ths->f_hat_iter, ths->alpha_iter, wh, (const R *)ths->p_hat_iter, N);
-    hxpawy<R>((R *)ths->r_iter, -ths->alpha_iter, nullptr, (const R *)ths->v_iter, M);
-    ths->dot_r_iter = (R)hdot<R>((const R *)ths->r_iter, w, M);
-    hcp_w<R>((R *)mv->f, w, (const R *)ths->r_iter, M);
+    hxpawy<R, K>((R *)ths->f_hat_iter, ths->alpha_iter, wh, (const R *)ths->p_hat_iter, N);
+    hxpawy<R, K>((R *)ths->r_iter, -ths->alpha_iter, nullptr, (const R *)ths->v_iter, M);
+    ths->dot_r_iter = (R)hdot<R, K>((const R *)ths->r_iter, w, M);
+    hcp_w<R, K>((R *)mv->f, w, (const R *)ths->r_iter, M);
     NFFT_HIP_CSWAP(ths->z_hat_iter, mv->f_hat);
     mv->mv_adjoint(mv);
     NFFT_HIP_CSWAP(ths->z_hat_iter, mv->f_hat);
     ths->dot_z_hat_iter_old = ths->dot_z_hat_iter;
-    ths->dot_z_hat_iter = (R)hdot<R>((const R *)ths->z_hat_iter, wh, N);
+    ths->dot_z_hat_iter = (R)hdot<R, K>((const R *)ths->z_hat_iter, wh, N);
     ths->beta_iter = ths->dot_z_hat_iter / ths->dot_z_hat_iter_old;
-    haxpy<R>((R *)ths->p_hat_iter, ths->beta_iter, (const R *)ths->z_hat_iter, N);
+    haxpy<R, K>((R *)ths->p_hat_iter, ths->beta_iter, (const R *)ths->z_hat_iter, N);
   }
 
   static void host_cgne(SPLAN *ths) {
@@ -459,23 +478,23 @@ struct SolverApi {
     const R *wh = ths->flags & PRECOMPUTE_DAMP ? ths->w_hat : nullptr;
     const R *w = ths->flags & PRECOMPUTE_WEIGHT ? ths->w : nullptr;
     ths->alpha_iter = ths->dot_r_iter / ths->dot_p_hat_iter;
-    hxpawy<R>((R *)ths->f_hat_iter, ths->alpha_iter, wh, (const R *)ths->p_hat_iter, N);
-    hcp_w<R>((R *)mv->f_hat, wh, (const R *)ths->p_hat_iter, N);
+    hxpawy<R, K>((R *)ths->f_hat_iter, ths->alpha_iter, wh, (const R *)ths->p_hat_iter, N);
+    hcp_w<R, K>((R *)mv->f_hat, wh, (const R *)ths->p_hat_iter, N);
     mv->mv_trafo(mv);
-    hxpawy<R>((R *)ths->r_iter, -ths->alpha_iter, nullptr, (const R *)mv->f, M);
+    hxpawy<R, K>((R *)ths->r_iter, -ths->alpha_iter, nullptr, (const R *)mv->f, M);
     ths->dot_r_iter_old = ths->dot_r_iter;
-    ths->dot_r_iter = (R)hdot<R>((const R *)ths->r_iter, w, M);
+    ths->dot_r_iter = (R)hdot<R, K>((const R *)ths->r_iter, w, M);
     ths->beta_iter = ths->dot_r_iter / ths->dot_r_iter_old;
-    hcp_w<R>((R *)mv->f, w, (const R *)ths->r_iter, M);
+    hcp_w<R, K>((R *)mv->f, w, (const R *)ths->r_iter, M);
     mv->mv_adjoint(mv);
-    haxpy<R>((R *)ths->p_hat_iter, ths->beta_iter, (const R *)mv->f_hat, N);
-    ths->dot_p_hat_iter = (R)hdot<R>((const R *)ths->p_hat_iter, wh, N);
+    haxpy<R, K>((R *)ths->p_hat_iter, ths->beta_iter, (const R *)mv->f_hat, N);
+    ths->dot_p_hat_iter = (R)hdot<R, K>((const R *)ths->p_hat_iter, wh, N);
   }
 #undef NFFT_HIP_CSWAP
 
   static void before_loop(SPLAN *ths) {
     clear_launch_error();
-    if (DevCgnr<R> *dv = find_dev<R>(ths)) dev_before_loop(ths, dv);
+    if (DevCgnr<R> *dv = K == 2 ? find_dev<R>(ths) : nullptr) dev_before_loop(ths, dv);
     else host_before_loop(ths);
   }
 
@@ -484,7 +503,7 @@ struct SolverApi {
     if (ths->flags & LANDWEBER) host_landweber(ths);
     if (ths->flags & STEEPEST_DESCENT) host_steepest(ths);
     if (ths->flags & CGNR) {
-      if (DevCgnr<R> *dv = find_dev<R>(ths)) dev_step(ths, dv);
+      if (DevCgnr<R> *dv = K == 2 ? find_dev<R>(ths) : nullptr) dev_step(ths, dv);
       else host_cgnr(ths);
     }
     if (ths->flags & CGNE) host_cgne(ths);
@@ -502,25 +521,25 @@ struct SolverApi {
         rg.map.erase(it);
       }
     }
-    if (ths->flags & PRECOMPUTE_WEIGHT) host_array_free<R>(ths->w);
-    if (ths->flags & PRECOMPUTE_DAMP) host_array_free<R>(ths->w_hat);
+    if (ths->flags & PRECOMPUTE_WEIGHT) vfree(ths->w);
+    if (ths->flags & PRECOMPUTE_DAMP) vfree(ths->w_hat);
     if (ths->flags & CGNR) {
-      host_array_free<R>(ths->v_iter);
-      host_array_free<R>(ths->z_hat_iter);
+      vfree(ths->v_iter);
+      vfree(ths->z_hat_iter);
     }
-    if (ths->flags & STEEPEST_DESCENT) host_array_free<R>(ths->v_iter);
-    host_array_free<R>(ths->p_hat_iter);
-    host_array_free<R>(ths->f_hat_iter);
-    host_array_free<R>(ths->r_iter);
-    host_array_free<R>(ths->y);
+    if (ths->flags & STEEPEST_DESCENT) vfree(ths->v_iter);
+    vfree(ths->p_hat_iter);
+    vfree(ths->f_hat_iter);
+    vfree(ths->r_iter);
+    vfree(ths->y);
   }
 
   static void set_sync(SPLAN *ths, int level) {
-    if (DevCgnr<R> *dv = find_dev<R>(ths)) dv->sync_level = level < 0 ? 0 : level > 2 ? 2 : level;
+    if (DevCgnr<R> *dv = K == 2 ? find_dev<R>(ths) : nullptr) dv->sync_level = level < 0 ? 0 : level > 2 ? 2 : level;
   }
 
   static void sync_all(SPLAN *ths) {
-    if (DevCgnr<R> *dv = find_dev<R>(ths)) {
+    if (DevCgnr<R> *dv = K == 2 ? find_dev<R>(ths) : nullptr) {
       int keep = dv->sync_level;
       dv->sync_level = 2;
       sync_out(ths, dv);
@@ -548,3 +567,17 @@ using nfft_hip::SolverApi;
 
 NFFT_HIP_EXPORT_SOLVER(solverf_, float, solverf_plan_complex, nfftf_mv_plan_complex, fftwf_complex)
 NFFT_HIP_EXPORT_SOLVER(solver_, double, solver_plan_complex, nfft_mv_plan_complex, fftw_complex)
+
+#define NFFT_HIP_EXPORT_SOLVER_DOUBLE(S, R, SPLAN, MV)                                          \
+  extern "C" {                                                                                  \
+  void S##init_advanced_double(SPLAN *ths, MV *mv, unsigned flags) {                            \
+    SolverApi<R, SPLAN, MV, R>::init_advanced(ths, mv, flags);                                  \
+  }                                                                                             \
+  void S##init_double(SPLAN *ths, MV *mv) { SolverApi<R, SPLAN, MV, R>::init_advanced(ths, mv, CGNR); } \
+  void S##before_loop_double(SPLAN *ths) { SolverApi<R, SPLAN, MV, R>::before_loop(ths); }      \
+  void S##loop_one_step_double(SPLAN *ths) { SolverApi<R, SPLAN, MV, R>::loop_one_step(ths); }  \
+  void S##finalize_double(SPLAN *ths) { SolverApi<R, SPLAN, MV, R>::finalize(ths); }            \
+  }
+
+NFFT_HIP_EXPORT_SOLVER_DOUBLE(solverf_, float, solverf_plan_double, nfftf_mv_plan_double)
+NFFT_HIP_EXPORT_SOLVER_DOUBLE(solver_, double, solver_plan_double, nfft_mv_plan_double)

@@ -82,6 +82,124 @@ NFFT_INT next_pow2(NFFT_INT x) {
   return p;
 }
 
+// ---- damping factors (reference kernel/util/damp.c:25-142) ----
+// Each "modified" factor of index kk is the sum of a kernel's Fourier
+// coefficients at kk and kk + 1 (the two integer frequencies around the
+// half-integer kk + 1/2).
+
+// sum_j a[j] k^j by Horner, degree D
+template <typename R, int D>
+R horner(const R (&a)[D + 1], R k) {
+  R v = a[D];
+  for (int j = D - 1; j >= 0; j--) v = v * k + a[j];
+  return v;
+}
+
+// Fourier coefficient at |k| of the Jackson kernel (sin(n x/2) / sin(x/2))^4,
+// normalised to 1 at k = 0: a cubic on [0, n), (2n-k)((2n-k)^2-1) / (2n(2n^2+1))
+// on [n, 2n].
+template <typename R>
+R jackson2_coef(R n, R k) {
+  const R den = R(2) * n * (R(2) * n * n + R(1));
+  if (k < n) {
+    const R a[4] = {0, R(3), R(6) * n, R(-3)};
+    return R(1) - horner<R, 3>(a, k) / den;
+  }
+  const R u = R(2) * n - k;
+  return u * (u * u - R(1)) / den;
+}
+
+// Fourier coefficient at |k| of the generalised Jackson kernel
+// (sin(n x/2) / sin(x/2))^8, normalised to 1 at k = 0: a degree-7 polynomial
+// in k on each of [0, n), [n, 2n), [2n, 3n), [3n, 4n), zero beyond.
+template <typename R>
+R jackson4_coef(R n, R k) {
+  const R n2 = n * n, n3 = n2 * n, n4 = n3 * n, n5 = n4 * n, n6 = n5 * n, n7 = n6 * n;
+  const R norm = R(2416) * n7 + R(1120) * n5 + R(784) * n3 + R(720) * n;
+  if (k < n) {
+    const R a[8] = {0,
+                    R(1260),
+                    R(1680) * n5 + R(2240) * n3 + R(2940) * n,
+                    R(-1715),
+                    -(R(560) * n3 + R(1400) * n),
+                    R(490),
+                    R(140) * n,
+                    R(-35)};
+    return R(1) - horner<R, 7>(a, k) / norm;
+  }
+  if (k < R(2) * n) {
+    const R a[8] = {R(2472) * n7 + R(336) * n5 + R(3528) * n3 - R(1296) * n,
+                    -(R(392) * n6 - R(3920) * n4 + R(8232) * n2 - R(756)),
+                    -(R(504) * n5 + R(10080) * n3 - R(5292) * n),
+                    -(R(1960) * n4 - R(7840) * n2 + R(1029)),
+                    R(2520) * n3 - R(2520) * n,
+                    -(R(1176) * n2 - R(294)),
+                    R(252) * n,
+                    R(-21)};
+    return horner<R, 7>(a, k) / norm;
+  }
+  if (k < R(3) * n) {
+    const R a[8] = {-(R(1112) * n7 - R(12880) * n5 + R(7448) * n3 - R(720) * n),
+                    R(12152) * n6 - R(27440) * n4 + R(8232) * n2 - R(252),
+                    -(R(19320) * n5 - R(21280) * n3 + R(2940) * n),
+                    R(13720) * n4 - R(7840) * n2 + R(343),
+                    -(R(5320) * n3 - R(1400) * n),
+                    R(1176) * n2 - R(98),
+                    R(-140) * n,
+                    R(7)};
+    return horner<R, 7>(a, k) / norm;
+  }
+  if (k < R(4) * n) {
+    const R u = R(4) * n - k, u2 = u * u;
+    return u * (u2 - R(1)) * (u2 - R(4)) * (u2 - R(9)) / norm;
+  }
+  return R(0);
+}
+
+template <typename R>
+R damp_fejer(NFFT_INT N, NFFT_INT kk) {
+  const R NN = (R)N;
+  return R(2) / (NN * NN) * (R(1) - std::fabs(R(2) * (R)kk + R(1)) / NN);
+}
+
+template <typename R>
+R damp_jackson2(NFFT_INT N, NFFT_INT kk) {
+  const R n = ((R)N / R(2) + R(1)) / R(2);
+  R s = 0;
+  for (NFFT_INT k = kk; k <= kk + 1; k++) s += jackson2_coef<R>(n, (R)(k < 0 ? -k : k));
+  return s;
+}
+
+template <typename R>
+R damp_jackson4(NFFT_INT N, NFFT_INT kk) {
+  const R n = ((R)N / R(2) + R(3)) / R(4);
+  R s = 0;
+  for (NFFT_INT k = kk; k <= kk + 1; k++) s += jackson4_coef<R>(n, (R)(k < 0 ? -k : k));
+  return s;
+}
+
+// coefficients |k|^(-2 mu), 1 at k = 0
+template <typename R>
+R damp_sobolev(R mu, NFFT_INT kk) {
+  R s = 0;
+  for (NFFT_INT k = kk; k <= kk + 1; k++) {
+    const NFFT_INT a = k < 0 ? -k : k;
+    s += a == 0 ? R(1) : std::pow((R)a, R(-2) * mu);
+  }
+  return s;
+}
+
+// coefficients (k^2 + c^2)^(-mu)
+template <typename R>
+R damp_multiquadric(R mu, R c, NFFT_INT kk) {
+  R s = 0;
+  for (NFFT_INT k = kk; k <= kk + 1; k++) {
+    const NFFT_INT a = k < 0 ? -k : k;
+    s += std::pow((R)(a * a) + c * c, -mu);
+  }
+  return s;
+}
+
 }  // namespace
 
 extern "C" {
@@ -191,6 +309,23 @@ NFFT_HIP_MALLOC_API(nfft_)
 
 NFFT_HIP_UTIL_API(nfftf_, float, fftwf_complex)
 NFFT_HIP_UTIL_API(nfft_, double, fftw_complex)
+
+// ------------------------------------------------------------ damping factors
+#define NFFT_HIP_DAMP_API(P, R)                                                          \
+  R P##modified_fejer(const NFFT_INT N, const NFFT_INT kk) { return damp_fejer<R>(N, kk); } \
+  R P##modified_jackson2(const NFFT_INT N, const NFFT_INT kk) {                          \
+    return damp_jackson2<R>(N, kk);                                                     \
+  }                                                                                     \
+  R P##modified_jackson4(const NFFT_INT N, const NFFT_INT kk) {                          \
+    return damp_jackson4<R>(N, kk);                                                     \
+  }                                                                                     \
+  R P##modified_sobolev(const R mu, const NFFT_INT kk) { return damp_sobolev<R>(mu, kk); } \
+  R P##modified_multiquadric(const R mu, const R c, const NFFT_INT kk) {                 \
+    return damp_multiquadric<R>(mu, c, kk);                                             \
+  }
+
+NFFT_HIP_DAMP_API(nfftf_, float)
+NFFT_HIP_DAMP_API(nfft_, double)
 
 }  // extern "C"
 

@@ -62,7 +62,11 @@ LAYOUT_C = r"""
   F(T,z_hat_iter) F(T,p_hat_iter) F(T,v_iter) F(T,alpha_iter) F(T,beta_iter) F(T,dot_r_iter) \
   F(T,dot_r_iter_old) F(T,dot_z_hat_iter) F(T,dot_z_hat_iter_old) F(T,dot_p_hat_iter) \
   F(T,dot_v_iter) printf("%s sizeof %zu\n", #T, sizeof(T));
-int main(void) { PLAN(nfftf_plan) PLAN(nfft_plan) SOLV(solverf_plan_complex) SOLV(solver_plan_complex) return 0; }
+#define MV(T) F(T,N_total) F(T,M_total) F(T,f_hat) F(T,f) F(T,mv_trafo) F(T,mv_adjoint) \
+  printf("%s sizeof %zu\n", #T, sizeof(T));
+int main(void) { PLAN(nfftf_plan) PLAN(nfft_plan) SOLV(solverf_plan_complex) SOLV(solver_plan_complex)
+  SOLV(solverf_plan_double) SOLV(solver_plan_double) MV(nfftf_mv_plan_double) MV(nfft_mv_plan_double)
+  return 0; }
 """
 
 
@@ -73,7 +77,9 @@ def test_struct_layout_matches_c(tmp_path):
     subprocess.run(["gcc", "-std=c99", "-I", INC, str(src), "-o", str(exe)], check=True)
     lines = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
     types = {"nfftf_plan": nf.nfftf_plan, "nfft_plan": nf.nfft_plan,
-             "solverf_plan_complex": nf.solverf_plan_complex, "solver_plan_complex": nf.solver_plan_complex}
+             "solverf_plan_complex": nf.solverf_plan_complex, "solver_plan_complex": nf.solver_plan_complex,
+             "solverf_plan_double": nf.solverf_plan_double, "solver_plan_double": nf.solver_plan_double,
+             "nfftf_mv_plan_double": nf.nfftf_mv_plan_double, "nfft_mv_plan_double": nf.nfft_mv_plan_double}
     n = 0
     for line in lines:
         if not line.strip():
@@ -84,7 +90,7 @@ def test_struct_layout_matches_c(tmp_path):
         else:
             assert getattr(types[T], field).offset == int(val), (T, field, val)
         n += 1
-    assert n == 2 * 31 + 2 * 19
+    assert n == 2 * 31 + 4 * 19 + 2 * 7
 
 
 def test_c_caller_compiles_and_links(tmp_path, libpath):

@@ -8,6 +8,9 @@ Pins, per oracle function:
                    (tests/nfft.c:217-284) for m = 4, 6, 8; adjointness
   fft              numpy.fft and a brute-force DFT
   next_power_of_2  the naive loop of tests/util.c:100-152
+  phi (sin branch) the analytic continuation of the sinh branch, evaluated in
+                   complex arithmetic (the branch lies below the golden files'
+                   resolution)
 and a planted-mistake test: deliberately wrong builds of the oracle
 (truncated window, missing deconvolution factor, shifted tap window, flipped
 FFT sign) must FAIL the golden checks.
@@ -136,6 +139,46 @@ def test_phi_branches_closed_form():
     assert O.phi_hut(n, 0, m, b) == pytest.approx(O.bessel_i0(m * b), rel=1e-15)
 
 
+def _phi_continuation(y, m, b):
+    """sinh(b s) / (pi s), s = sqrt(m^2 - y^2), with complex s: the function
+    is entire in s^2 (sinh(b s)/s is even in s), so for |y| > m it continues
+    analytically to sin(b t) / (pi t), t = sqrt(y^2 - m^2) -- the second
+    branch of the reference's window (include/infft.h:209-215) is fixed by
+    the first."""
+    s = np.sqrt(np.complex128(m * m - y * y))
+    v = np.sinh(b * s) / (np.pi * s)
+    assert abs(v.imag) <= 1e-12 * max(1.0, abs(v.real))
+    return float(v.real)
+
+
+@pytest.mark.parametrize("m", [2, 4, 6, 8])
+@pytest.mark.parametrize("sigma", [1.5, 2.0, 3.0])
+def test_phi_sin_branch_is_analytic_continuation(m, sigma):
+    """Pins the window's sin branch (|y| > m, the two outermost taps of every
+    node) against the analytic continuation of its sinh branch, evaluated
+    in complex arithmetic by numpy -- and shows that a build that truncates
+    the window at |y| = m (planted mistake 1) fails this pin."""
+    n = 64
+    b = O.kb_b(sigma)
+    ys = np.concatenate([np.linspace(m + 1e-3, m + 1.0, 41), -np.linspace(m + 1e-3, m + 1.0, 41)])
+    ref = np.array([_phi_continuation(y, m, b) for y in ys])
+    got = np.array([O.phi(n, y / n, m, b) for y in ys])
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+    assert np.max(np.abs(ref)) > 0.05  # the branch is not negligible in absolute terms
+    # inside the support the same expression is the sinh branch itself
+    yi = np.linspace(-m + 1e-3, m - 1e-3, 33)
+    gi = np.array([O.phi(n, y / n, m, b) for y in yi])
+    ri = np.array([_phi_continuation(y, m, b) for y in yi])
+    assert np.max(np.abs(gi - ri) / np.abs(ri)) <= 1e-13
+    subprocess.run(["make", "-s", "-C", O._HERE, "liboracle_plant1.so"], check=True)
+    O.use_library(os.path.join(O._HERE, "liboracle_plant1.so"))
+    try:
+        bad = np.array([O.phi(n, y / n, m, b) for y in ys])
+    finally:
+        O.use_library(None)
+    assert np.max(np.abs(bad - ref)) > 1e3 * 1e-13 * np.max(np.abs(ref))
+
+
 def _golden_pass_rate(libname):
     """Fraction of fast-path golden checks (m = 6, double) a build passes."""
     O.use_library(os.path.join(O._HERE, libname))
@@ -170,10 +213,9 @@ def test_planted_mistakes_are_caught(plant):
 
 
 def test_window_outer_taps_below_golden_resolution():
-    """Documents a limit of the pins: zeroing the two outermost taps (the sin
-    branch of include/infft.h:209-215) changes results by less than the
-    golden vectors resolve (relative ~1e-8 at m=4, ~1e-13 at m=6), so that
-    branch of the oracle is parity-unpinned; it follows the reference source
-    text.  If this test ever fails, a sharper pin has appeared -- use it."""
+    """Documents a limit of the golden files: zeroing the two outermost taps
+    (the sin branch of include/infft.h:209-215) changes results by less than
+    they resolve (relative ~1e-8 at m=4, ~1e-13 at m=6).  The branch is
+    pinned instead by test_phi_sin_branch_is_analytic_continuation."""
     subprocess.run(["make", "-s", "-C", O._HERE, "liboracle_plant1.so"], check=True)
     assert _golden_pass_rate("liboracle_plant1.so") == 1.0
