@@ -26,9 +26,16 @@
 
 #ifndef FFTW3_H
 /* Layout of FFTW's own definitions: two reals, re then im; plans are opaque
- * pointers. */
+ * pointers.  As in FFTW's header, a C caller that included <complex.h> first
+ * gets the C99 complex types (same size and layout), so code such as
+ * p->f_hat[k] = c * cexp(...) compiles as it does against the reference. */
+#if !defined(FFTW_NO_Complex) && defined(_Complex_I) && defined(complex) && defined(I)
+typedef double _Complex fftw_complex;
+typedef float _Complex fftwf_complex;
+#else
 typedef double fftw_complex[2];
 typedef float fftwf_complex[2];
+#endif
 typedef struct fftw_plan_s *fftw_plan;
 typedef struct fftwf_plan_s *fftwf_plan;
 #endif

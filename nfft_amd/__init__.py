@@ -202,10 +202,10 @@ class Plan:
     """An nfft_plan (precision 'double') or nfftf_plan ('single') driven
     through the C ABI.  Construction calls ``init_guru`` (or ``init`` when
     ``n``, ``m`` and ``flags`` are all left out, ``init_1d/2d/3d`` with
-    ``how='init_Nd'``)."""
+    ``how='init_Nd'``, ``init_lin`` with ``how='init_lin'`` and ``lin_K``)."""
 
     def __init__(self, N, M, n=None, m=None, flags=None, precision="double",
-                 fftw_flags=FFTW_ESTIMATE | FFTW_DESTROY_INPUT, how=None):
+                 fftw_flags=FFTW_ESTIMATE | FFTW_DESTROY_INPUT, how=None, lin_K=1 << 20):
         self.L = lib()
         self.precision = precision
         self.pre = "nfft_" if precision == "double" else "nfftf_"
@@ -218,6 +218,16 @@ class Plan:
         Na = (ctypes.c_int * d)(*N)
         if how == "init_Nd":
             getattr(self.L, self.pre + "init_%dd" % d)(self.pp, *N, int(M))
+        elif how == "init_lin":
+            # init_lin(plan, d, N, M, n, m, K, flags, fftw_flags): PRE_LIN_PSI
+            # with a table of K window samples per unit
+            n = n or [2 * _next_pow2(t) for t in N]
+            m = m or (8 if precision == "double" else 4)
+            if flags is None:
+                flags = PRE_PHI_HUT | PRE_LIN_PSI | MALLOC_X | MALLOC_F_HAT | MALLOC_F | FFTW_INIT | FFT_OUT_OF_PLACE
+            na = (ctypes.c_int * d)(*[int(t) for t in n])
+            self._fn("init_lin")(self.pp, d, Na, int(M), na, int(m), int(lin_K), int(flags),
+                                 int(fftw_flags))
         elif how == "init" or (n is None and m is None and flags is None):
             self._fn("init")(self.pp, d, Na, int(M))
         else:

@@ -2,7 +2,8 @@
 
 Mirrors tests/nfft.c of the reference: every golden file (Mathematica NDFT at
 64 digits) crossed with the plan initialisers the reference uses
-(init_1d/2d/3d, init, init_guru with PRE_PSI, init_guru with PRE_FULL_PSI)
+(init_1d/2d/3d, init, init_guru with PRE_PSI, PRE_FULL_PSI, PRE_LIN_PSI and
+FG_PSI | PRE_FG_PSI, init_lin)
 and the entry points (trafo, trafo_Nd, adjoint, adjoint_Nd, trafo_direct,
 adjoint_direct), for both precision families.  Error measures
 (tests/nfft.c:739-781): trafo ||f - f_ref||_inf / ||f_hat||_1, adjoint
@@ -28,6 +29,13 @@ INITS = {
     "init": dict(how="init"),
     "guru_pre_psi": dict(flags=nf.PRE_PHI_HUT | nf.PRE_PSI | DEFAULT),
     "guru_pre_full_psi": dict(flags=nf.PRE_PHI_HUT | nf.PRE_FULL_PSI | DEFAULT),
+    # the reference's suite lists these too (tests/nfft.c:697-707; PRE_LIN_PSI
+    # commented out there, PRE_FG_PSI for its Gaussian window only); here the
+    # window is evaluated in the kernels for every PRE_* flag, so they must
+    # meet the same bound
+    "guru_pre_lin_psi": dict(flags=nf.PRE_PHI_HUT | nf.PRE_LIN_PSI | DEFAULT),
+    "guru_pre_fg_psi": dict(flags=nf.PRE_PHI_HUT | nf.FG_PSI | nf.PRE_FG_PSI | DEFAULT),
+    "init_lin": dict(how="init_lin", flags=nf.PRE_PHI_HUT | nf.PRE_LIN_PSI | DEFAULT),
 }
 
 
@@ -38,6 +46,8 @@ def _run(name, precision, init, entry):
     kw = dict(INITS[init])
     if "flags" in kw:
         kw["n"] = [2 * O.next_power_of_2(t) for t in N]
+        if kw.get("how") == "init_lin":
+            kw["lin_K"] = 1 << 20  # nfft_check needs K >= M
         kw["m"] = 8 if precision == "double" else 4
     p = nf.Plan(N, M, precision=precision, **kw)
     try:

@@ -5,7 +5,8 @@ against least squares on CPU) runs the same recurrences with the FP64 NFFT
 oracle as operator.  The library's device CGNR (plan is one of its own) and
 its host path (the other methods) must follow the same iterates:
   double family: f_hat_iter and dot_* within 1e-9 relative for 8 steps
-  float family:  within 2e-4 relative (FP32 operator, FP64 inner products)
+  float family:  within 2e-4 relative (FP32 operator, FP64 inner products;
+                 2e-3 with the ill-conditioning Jackson damping factors)
 """
 import numpy as np
 import pytest
@@ -111,7 +112,11 @@ def test_damped_solver_with_jackson_factors(method, precision):
     assert wh.min() > 0 and wh.max() <= 4.0
     res = run_case(method, nf.PRECOMPUTE_WEIGHT | nf.PRECOMPUTE_DAMP, precision, steps=8, M=M,
                    w_hat=wh)
-    tol = 1e-9 if precision == "double" else 2e-4
+    # the Jackson factors span 5e-5 .. 3.8, so the damped normal operator is
+    # far worse conditioned than with the U[0.5, 2] factors above; the FP32
+    # transforms' ~1e-6 relative error grows through the recurrence to
+    # ~6e-4 by step 8 (measured), hence the looser float tolerance
+    tol = 1e-9 if precision == "double" else 2e-3
     for it, (ef, er, err, err_y) in enumerate(res):
         # the FP32 residual, updated by recurrence, carries cancellation
         # error of the size of FP32 rounding of y: measured against |y|

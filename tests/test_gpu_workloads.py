@@ -1,0 +1,241 @@
+"""Parity on the batched and solver workloads at their full sizes (GPU).
+
+* Coils: 2-D N=256^2, n=512^2, m=4, 403 x 512 golden-angle radial nodes
+  (M = 206,336), 32 right-hand sides.  The FP32 batched launches
+  (trafo_batch_device / adjoint_batch_device) and their hipGraph replays are
+  checked right-hand side by right-hand side against the FP64 plan
+  (rel l2 <= 1e-5), and the FP64 plan against the oracle on a 2000-node
+  sample (rel l2 <= 1e-12), as the full-size single-plan test does.
+* Many plans: 64 plans of N=128^2, n=256^2, m=4 with M log-uniform in
+  [1e4, 1e6] (iid uniform nodes), trafo and adjoint of all of them as one
+  hipGraph each; every plan's output against its FP64 plan (1e-5) and, on
+  500 sampled nodes, against the oracle.
+* Weighted CGNR at N=256^2 (sigma 2, m 4, FP32) on a held-out radial and a
+  held-out spiral trajectory with 1% noise: unweighted, Pipe-Menon and
+  learned weights; 8 device CGNR steps against the solver oracle driven by
+  the FP64 oracle NFFT (reference kernel/solver/solver.c:232-292).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import nfft_amd as nf
+import oracle as O
+from oracle import solver as S
+from nfft_amd import workloads as W
+
+torch = pytest.importorskip("torch")
+
+FLAGS = nf.PRE_PHI_HUT | nf.PRE_PSI | nf.MALLOC_X | nf.MALLOC_F_HAT | nf.MALLOC_F | nf.FFTW_INIT
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.complex128)
+    b = np.asarray(b, dtype=np.complex128)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def make_plan(N, x, precision, m=4):
+    p = nf.Plan(N, x.shape[0], n=[2 * t for t in N], m=m, flags=FLAGS, precision=precision)
+    p.x[:] = x
+    p.precompute()
+    return p
+
+
+def fp64_pair(p64, fh, f):
+    """FP64 plan: trafo of fh and adjoint of f through the host arrays."""
+    p64.f_hat[:] = fh
+    p64.trafo()
+    a = p64.f.astype(np.complex128).copy()
+    p64.f[:] = f
+    p64.adjoint()
+    return a, p64.f_hat.astype(np.complex128).copy()
+
+
+def oracle_sample_checks(N, n, m, x, fh, f, Af64, p64, nsample, seed):
+    """FP64 GPU plan against the oracle: trafo on sampled nodes (full-grid D
+    and FFT in the oracle), adjoint of an f supported on those nodes."""
+    M = x.shape[0]
+    idx = np.random.default_rng(seed).choice(M, size=nsample, replace=False)
+    g = O.fft(n, O.stage_D(N, n, m, fh), -1)
+    e_t = rel_l2(Af64[idx], O.stage_B(N, n, m, x[idx], g))
+    fs = np.zeros(M, dtype=np.complex128)
+    fs[idx] = f[idx]
+    p64.f[:] = fs
+    p64.adjoint()
+    e_a = rel_l2(p64.f_hat.astype(np.complex128), O.nfft_adjoint(N, n, m, x[idx], f[idx]))
+    return e_t, e_a
+
+
+@pytest.mark.gpu
+def test_coils_32_rhs_full_size():
+    N, n, m, R = [256, 256], [512, 512], 4, 32
+    x = W.golden_radial(403, 512)
+    M, NN = x.shape[0], 256 * 256
+    fh = np.stack([W.complex_normal(NN, seed=1000 + r) for r in range(R)])
+    f = np.stack([W.complex_normal(M, seed=2000 + r) for r in range(R)])
+    p = make_plan(N, x, "single")
+    try:
+        xs = p.x.astype(np.float64).copy()
+        d_fh = torch.from_numpy(fh.astype(np.complex64).ravel()).cuda()
+        d_f = torch.from_numpy(f.astype(np.complex64).ravel()).cuda()
+        t_out = torch.empty(R * M, dtype=torch.complex64, device="cuda")
+        a_out = torch.empty(R * NN, dtype=torch.complex64, device="cuda")
+        p.trafo_batch_device(d_fh.data_ptr(), t_out.data_ptr(), R)
+        p.adjoint_batch_device(d_f.data_ptr(), a_out.data_ptr(), R)
+        p.synchronize()
+        Af32 = t_out.cpu().numpy().reshape(R, M)
+        AHf32 = a_out.cpu().numpy().reshape(R, NN)
+        # graph replays of the same batch equal the eager launches bit for bit
+        g_t = torch.empty_like(t_out)
+        g_a = torch.empty_like(a_out)
+        gt = nf.Graph([p], [d_fh.data_ptr()], [g_t.data_ptr()], nrhs=R)
+        ga = nf.Graph([p], [d_f.data_ptr()], [g_a.data_ptr()], nrhs=R, adjoint=True)
+        try:
+            for rep in range(2):
+                g_t.zero_()
+                g_a.zero_()
+                torch.cuda.synchronize()
+                gt.launch()
+                ga.launch()
+                ga.synchronize()
+                assert torch.equal(g_t, t_out), rep
+                assert torch.equal(g_a, a_out), rep
+        finally:
+            gt.destroy()
+            ga.destroy()
+    finally:
+        p.finalize()
+
+    p64 = make_plan(N, xs, "double")
+    try:
+        worst_t = worst_a = 0.0
+        for r in range(R):
+            # the FP64 reference sees the FP32 inputs
+            fh_r = fh[r].astype(np.complex64).astype(np.complex128)
+            f_r = f[r].astype(np.complex64).astype(np.complex128)
+            Af64, AHf64 = fp64_pair(p64, fh_r, f_r)
+            worst_t = max(worst_t, rel_l2(Af32[r], Af64))
+            worst_a = max(worst_a, rel_l2(AHf32[r], AHf64))
+            if r in (0, R - 1):
+                e_t, e_a = oracle_sample_checks(N, n, m, xs, fh_r, f_r, Af64, p64, 2000, seed=r)
+                print("coil %d: FP64 vs oracle trafo %.2e adjoint %.2e" % (r, e_t, e_a))
+                assert e_t < 1e-12 and e_a < 1e-12
+        print("coils: worst FP32 vs FP64 over 32 rhs: trafo %.2e adjoint %.2e" % (worst_t, worst_a))
+        assert worst_t < 1e-5 and worst_a < 1e-5
+    finally:
+        p64.finalize()
+
+
+@pytest.mark.gpu
+def test_64_plans_one_graph():
+    N, n, m = [128, 128], [256, 256], 4
+    r = np.random.default_rng(4)
+    Ms = np.round(10 ** r.uniform(4, 6, 64)).astype(int)
+    NN = 128 * 128
+    plans, xs, fhs, fs, t_outs, a_outs = [], [], [], [], [], []
+    try:
+        for i, Mi in enumerate(Ms):
+            x = W.uniform(int(Mi), 2, seed=100 + i)
+            plans.append(make_plan(N, x, "single"))
+            xs.append(plans[-1].x.astype(np.float64).copy())
+            fhs.append(torch.from_numpy(W.complex_normal(NN, seed=300 + i).astype(np.complex64)).cuda())
+            fs.append(torch.from_numpy(W.complex_normal(int(Mi), seed=500 + i).astype(np.complex64)).cuda())
+            t_outs.append(torch.zeros(int(Mi), dtype=torch.complex64, device="cuda"))
+            a_outs.append(torch.zeros(NN, dtype=torch.complex64, device="cuda"))
+        gt = nf.Graph(plans, [t.data_ptr() for t in fhs], [t.data_ptr() for t in t_outs])
+        ga = nf.Graph(plans, [t.data_ptr() for t in fs], [t.data_ptr() for t in a_outs], adjoint=True)
+        try:
+            gt.launch()
+            ga.launch()
+            ga.synchronize()
+        finally:
+            gt.destroy()
+            ga.destroy()
+        t32 = [t.cpu().numpy() for t in t_outs]
+        a32 = [t.cpu().numpy() for t in a_outs]
+        fh_np = [t.cpu().numpy().astype(np.complex128) for t in fhs]
+        f_np = [t.cpu().numpy().astype(np.complex128) for t in fs]
+    finally:
+        for p in plans:
+            p.finalize()
+
+    worst_t = worst_a = worst_ot = worst_oa = 0.0
+    for i, Mi in enumerate(Ms):
+        p64 = make_plan(N, xs[i], "double")
+        try:
+            Af64, AHf64 = fp64_pair(p64, fh_np[i], f_np[i])
+            worst_t = max(worst_t, rel_l2(t32[i], Af64))
+            worst_a = max(worst_a, rel_l2(a32[i], AHf64))
+            e_t, e_a = oracle_sample_checks(N, n, m, xs[i], fh_np[i], f_np[i], Af64, p64, 500,
+                                            seed=i)
+            worst_ot, worst_oa = max(worst_ot, e_t), max(worst_oa, e_a)
+        finally:
+            p64.finalize()
+    print("64 plans (%d nodes): FP32 graph vs FP64 trafo %.2e adjoint %.2e; FP64 vs oracle "
+          "trafo %.2e adjoint %.2e" % (Ms.sum(), worst_t, worst_a, worst_ot, worst_oa))
+    assert worst_t < 1e-5 and worst_a < 1e-5
+    assert worst_ot < 1e-12 and worst_oa < 1e-12
+
+
+def _weights(kind, p):
+    from nfft_amd import density as D
+    from nfft_amd import dcf_train as T
+    if kind == "plain":
+        return None
+    if kind == "pipe_menon":
+        return D.pipe_menon(p, 20)
+    model = D.DcfModel.load(T.DEFAULT_MODEL)
+    try:
+        return model.weights(p).cpu().numpy().astype(np.float64)
+    finally:
+        model.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["plain", "pipe_menon", "learned"])
+@pytest.mark.parametrize("seed", [5001, 5006], ids=["radial", "spiral"])
+def test_weighted_cgnr_256(kind, seed):
+    from nfft_amd import dcf_train as T
+    if kind == "learned" and not os.path.exists(T.DEFAULT_MODEL):
+        pytest.skip("no trained model")
+    N, m = 256, 4
+    x, desc = T.trajectory(seed, N)
+    p = T.make_plan(x, N, m)
+    try:
+        M, NN = p.M, N * N
+        xr = p.x.astype(np.float64).copy()
+        n = p.n
+        f_true = W.phantom(N, 20, seed=seed)
+        y0 = O.nfft_trafo([N, N], n, m, xr, f_true.astype(np.complex128))
+        y = y0 + 0.01 * np.sqrt(np.mean(np.abs(y0) ** 2)) * W.complex_normal(M, seed=seed + 1)
+        w = _weights(kind, p)
+        flags = nf.CGNR | (nf.PRECOMPUTE_WEIGHT if w is not None else 0)
+        s = nf.Solver(p, flags)
+        o = S.Solver(lambda v: O.nfft_trafo([N, N], n, m, xr, v),
+                     lambda v: O.nfft_adjoint([N, N], n, m, xr, v), M, NN, flags)
+        try:
+            s.y[:] = y
+            o.y = s.y.astype(np.complex128).copy()
+            s.f_hat_iter[:] = 0
+            o.f_hat_iter = np.zeros(NN, complex)
+            if w is not None:
+                assert np.all(np.isfinite(w)) and w.min() > 0
+                s.w[:] = w
+                o.w = s.w.astype(np.float64).copy()
+            s.before_loop()
+            o.before_loop()
+            for it in range(8):
+                s.loop_one_step()
+                o.loop_one_step()
+                ef = rel_l2(s.f_hat_iter, o.f_hat_iter)
+                ez = abs(s.s.dot_z_hat_iter - o.dot_z_hat_iter) / abs(o.dot_z_hat_iter)
+                er = abs(s.s.dot_r_iter - o.dot_r_iter) / abs(o.dot_r_iter)
+                assert ef < 2e-4 and ez < 2e-4 and er < 2e-4, (desc, kind, it, ef, ez, er)
+            print("%s %s (M=%d): after 8 steps f_hat rel %.2e, dot_z rel %.2e"
+                  % (desc[0], kind, M, ef, ez))
+        finally:
+            s.finalize()
+    finally:
+        p.finalize()
