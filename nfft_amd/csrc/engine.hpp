@@ -117,6 +117,26 @@ template <typename R> struct FftPass {
   // batched launches (blockIdx.y = right-hand side): offsets of the grid and
   // of the coefficient vectors per right-hand side
   long bs_g, bs_c;
+  // Four-step split of an axis longer than one workgroup's LDS lines (or
+  // whose length has a factor no single stage handles), n = n1 n2 with the
+  // axis index j1 + n1 j2:
+  //   four = 1: lines (other coordinates, j1), elements j2, length n2; the
+  //             result k2 is multiplied by w_n^(j1 k2) and written back in
+  //             place (position j1 + n1 k2); the input's band test uses the
+  //             axis index j1 + n1 j2
+  //   four = 2: lines (other coordinates, k2), elements j1 at j1 + n1 k2,
+  //             length n1; the result k1 is the axis index k2 + n2 k1, where
+  //             it is written (to the other grid buffer) with the band rule
+  //             of out_mode
+  // four = 0: an ordinary pass (nsub = 1).
+  int four;
+  int nsub;     // lines per line of the other coordinates: n1 or n2
+  int nfull;    // n
+  int fn1, fn2;
+  const cplx<R> *twiddle_full;  // exp(-2 pi i k / n), k < n
+  // grid buffer the pass reads and the one it writes: 0 the plan's grid, 1
+  // its second grid (four-step plans only)
+  int src, dst;
 };
 
 template <typename R> struct Engine {
@@ -184,10 +204,9 @@ template <typename R> struct Engine {
   int max_per_sub = 1024;
   // B kernel shapes: 0 lane per node (lane), 1 wave per node (wave); for
   // spreading also 2 private tile per wavefront (priv), 3 row-partitioned
-  // tile (rows), and the FP32 3-D run-merging kernels 4 (runs), 5 (runs5),
-  // 6 (runs2s), 7 (runs2st), 8 (runs1w2s), 9 (runs1w3s), 10 (runs2sbc),
-  // 11 (runs1w2sbc), 12 (runs1w3sbc), 13 (runs2rs: one stream, runs load and
-  // store their tile cells), and the 2-D run-merging kernel 14 (runs2d).
+  // tile (rows), the FP32 3-D run-merging kernels 10 (runs2sbc: two node
+  // streams per wavefront) and 13 (runs2rs: one stream, runs load and store
+  // their tile cells), and the 2-D run-merging kernel 14 (runs2d).
   // Set at plan creation; NFFT_HIP_INTERP / NFFT_HIP_SPREAD = name override.
   int interp_variant = 0;
   int spread_variant = 1;
@@ -197,6 +216,13 @@ template <typename R> struct Engine {
   std::vector<FftAxis<R>> axes;
   std::vector<FftPass<R>> fwd_passes;  // trafo: D fused into the first pass
   std::vector<FftPass<R>> bwd_passes;  // adjoint: D^T fused into the last pass
+  // four-step axes: their twiddle tables (sub-lengths and full length) and
+  // the second grid their passes write; trafo_grid = the buffer holding the
+  // trafo's transformed grid (0 g, 1 g2)
+  std::vector<cplx<R> *> extra_twiddles;
+  cplx<R> *g2 = nullptr;
+  int trafo_grid = 0;
+  cplx<R> *grid_after_trafo() const { return trafo_grid ? g2 : g; }
 
   void create(int d_, const long *N_, const long *n_, long M_, int m_);
   void destroy();
@@ -219,6 +245,8 @@ template <typename R> struct Engine {
   void reserve_batch(int nrhs);
   void trafo_batch(const cplx<R> *fhat_dev, cplx<R> *f_dev, int nrhs);
   void adjoint_batch(const cplx<R> *f_dev, cplx<R> *fhat_dev, int nrhs);
+  void adjoint_batch_chunks(const cplx<R> *f_dev, cplx<R> *fhat_dev, int nrhs, int k,
+                            cplx<R> *grids, cplx<R> *tls);
   cplx<R> *bgrid = nullptr;   // batched grids
   cplx<R> *btiles = nullptr;  // batched spreading tiles
   long bgrid_cap = 0, btiles_cap = 0;  // in complex values
@@ -253,7 +281,7 @@ void launch_deconv_from_grid(const cplx<R> *g, cplx<R> *fhat, const R *phi_inv, 
 // pass's / geometry's batch strides; gb / tiles override e.geom / e.tiles
 template <typename R>
 void launch_fft_pass(cplx<R> *g, const cplx<R> *fhat_in, cplx<R> *fhat_out, const FftPass<R> &p,
-                     int sign, hipStream_t s, int nb = 1);
+                     int sign, hipStream_t s, int nb = 1, cplx<R> *g2 = nullptr);
 template <typename R>
 void launch_interp(const cplx<R> *g, cplx<R> *f, const Engine<R> &e, hipStream_t s, int nb = 1,
                    const BinGeom *gb = nullptr);

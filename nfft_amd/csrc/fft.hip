@@ -207,12 +207,15 @@ __device__ __forceinline__ int band_coord(int k, int n, int N) {
 
 constexpr int kMaxLinesPerBlock = 256;
 
+// gsrc / gdst: the grid buffer the pass reads and the one it writes (the
+// same buffer except in the second half of a four-step axis)
 template <typename R, int SIGN, int CAP, bool POW2>
-__global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
+__global__ __launch_bounds__(256) void k_fft_pass(const cplx<R> *gsrc, cplx<R> *gdst,
                                                   const cplx<R> *__restrict__ fin,
                                                   cplx<R> *__restrict__ fout, FftPass<R> P) {
   // right-hand side blockIdx.y of a batched launch
-  g += (long)blockIdx.y * P.bs_g;
+  gsrc += (long)blockIdx.y * P.bs_g;
+  gdst += (long)blockIdx.y * P.bs_g;
   if (fin) fin += (long)blockIdx.y * P.bs_c;
   if (fout) fout += (long)blockIdx.y * P.bs_c;
   extern __shared__ unsigned char fft_smem[];
@@ -228,6 +231,7 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
   __shared__ long s_goff[kMaxLinesPerBlock];
   __shared__ long s_foff[kMaxLinesPerBlock];
   __shared__ R s_fac[kMaxLinesPerBlock];
+  __shared__ int s_sub[kMaxLinesPerBlock];
   const int n = P.len;
   const long L0 = (long)blockIdx.x * P.lines_per_block;
   const int TL = (int)min((long)P.lines_per_block, P.nlines - L0);
@@ -238,6 +242,13 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
   // other dimensions
   for (int ll = threadIdx.x; ll < TL; ll += blockDim.x) {
     long L = L0 + ll;
+    // four-step halves: the sub-line index (j1 or k2) runs fastest
+    int sub = 0;
+    if (P.four) {
+      sub = (int)(L % P.nsub);
+      L /= P.nsub;
+    }
+    s_sub[ll] = sub;
     int c[kMaxDim - 1] = {};
     for (int s = P.nod - 1; s >= 0; s--) {
       c[s] = (int)(L % P.cnt[s]);
@@ -267,7 +278,20 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
   // across adjacent lines otherwise.  Each thread issues all its loads
   // (at most CAP / 256) before the first LDS store, so their latencies
   // overlap instead of adding up.
-  const bool contiguous = P.stride == 1;
+  // element k of line ll: its position along the axis in memory and, for
+  // the band tests, its axis index (the two differ only in four-step halves)
+  const int n1 = P.fn1, n2 = P.fn2;
+  auto in_pos = [&](int ll, int k) {
+    return P.four == 0 ? k : (P.four == 1 ? s_sub[ll] + n1 * k : k + n1 * s_sub[ll]);
+  };
+  auto out_pos = [&](int ll, int k) {
+    return P.four == 0 ? k : (P.four == 1 ? s_sub[ll] + n1 * k : s_sub[ll] + n2 * k);
+  };
+  const int nax = P.four ? P.nfull : n;
+  // a wave walks along a line where its elements are adjacent in memory,
+  // across lines otherwise
+  const bool contiguous = P.stride == 1 && P.four != 1;
+  const bool contiguous_out = P.stride == 1 && P.four == 0;
   constexpr int PER = CAP / 256;
   cplx<R> v[PER];
   int dst[PER];
@@ -286,13 +310,14 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
         ll = qd - k * TL;
       }
       dst[u] = lds_idx(ll, k, ls);
+      const int kp = in_pos(ll, k);
       if (P.in_mode == kFromGrid) {
-        v[u] = g[s_goff[ll] + (long)k * P.stride];
+        v[u] = gsrc[s_goff[ll] + (long)kp * P.stride];
       } else {
-        const int ks = band_coord(k, n, P.Nband);
+        const int ks = band_coord(kp, nax, P.Nband);
         if (ks >= 0) {
           if (P.in_mode == kFromGridBand) {
-            v[u] = g[s_goff[ll] + (long)k * P.stride];
+            v[u] = gsrc[s_goff[ll] + (long)kp * P.stride];
           } else {
             const cplx<R> w = fin[s_foff[ll] + (long)ks * P.fstr_ax];
             const R f = s_fac[ll] * P.phi_ax[ks];
@@ -337,21 +362,28 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<R> *__restrict__ g,
     const int qd = threadIdx.x + u * 256;
     if (qd >= tot) break;
     int ll, k;
-    if (contiguous) {
+    if (contiguous_out) {
       ll = qd / n;
       k = qd - ll * n;
     } else {
       k = qd / TL;
       ll = qd - k * TL;
     }
-    const cplx<R> w = buf[lds_idx(ll, k, ls)];
+    cplx<R> w = buf[lds_idx(ll, k, ls)];
+    if (P.four == 1) {
+      // w_n^(j1 k2), j1 k2 < n
+      cplx<R> t = P.twiddle_full[s_sub[ll] * k];
+      if (SIGN > 0) t.y = -t.y;
+      w = cmul<R>(w, t);
+    }
+    const int kp = out_pos(ll, k);
     if (P.out_mode == kToGrid) {
-      g[s_goff[ll] + (long)k * P.stride] = w;
+      gdst[s_goff[ll] + (long)kp * P.stride] = w;
     } else {
-      const int ks = band_coord(k, n, P.Nband);
+      const int ks = band_coord(kp, nax, P.Nband);
       if (ks >= 0) {
         if (P.out_mode == kToGridBand) {
-          g[s_goff[ll] + (long)k * P.stride] = w;
+          gdst[s_goff[ll] + (long)kp * P.stride] = w;
         } else {
           const R f = s_fac[ll] * P.phi_ax[ks];
           fout[s_foff[ll] + (long)ks * P.fstr_ax] = cplx<R>{w.x * f, w.y * f};
@@ -366,7 +398,7 @@ template <typename R> constexpr int fft_cap() { return sizeof(R) == 4 ? 4096 : 2
 
 template <typename R>
 void launch_fft_pass(cplx<R> *g, const cplx<R> *fhat_in, cplx<R> *fhat_out, const FftPass<R> &p0,
-                     int sign, hipStream_t s, int nb) {
+                     int sign, hipStream_t s, int nb, cplx<R> *g2) {
   if (p0.nlines <= 0) return;
   // Fewer lines per workgroup than fit in LDS when the launch would leave
   // the GPU short of workgroups (small grids: aim for 1024 of them over all
@@ -389,13 +421,16 @@ void launch_fft_pass(cplx<R> *g, const cplx<R> *fhat_in, cplx<R> *fhat_out, cons
   for (int i = 0; i < p.nstages; i++) pow2 = pow2 && (p.radix[i] == 2 || p.radix[i] == 4 || p.radix[i] == 8);
   auto fn = sign < 0 ? (pow2 ? k_fft_pass<R, -1, CAP, true> : k_fft_pass<R, -1, CAP, false>)
                      : (pow2 ? k_fft_pass<R, +1, CAP, true> : k_fft_pass<R, +1, CAP, false>);
-  hipLaunchKernelGGL(fn, dim3((unsigned)blocks, (unsigned)nb), dim3(256), lds, s, g, fhat_in,
-                     fhat_out, p);
+  cplx<R> *src = p.src ? g2 : g;
+  cplx<R> *dst = p.dst ? g2 : g;
+  hipLaunchKernelGGL(fn, dim3((unsigned)blocks, (unsigned)nb), dim3(256), lds, s,
+                     (const cplx<R> *)src, dst, fhat_in, fhat_out, p);
 }
 
 template void launch_fft_pass<float>(cplx<float> *, const cplx<float> *, cplx<float> *,
-                                     const FftPass<float> &, int, hipStream_t, int);
+                                     const FftPass<float> &, int, hipStream_t, int, cplx<float> *);
 template void launch_fft_pass<double>(cplx<double> *, const cplx<double> *, cplx<double> *,
-                                      const FftPass<double> &, int, hipStream_t, int);
+                                      const FftPass<double> &, int, hipStream_t, int,
+                                      cplx<double> *);
 
 }  // namespace nfft_hip

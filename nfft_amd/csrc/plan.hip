@@ -336,17 +336,9 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   auto variant = [](const char *name, int dflt) {
     const char *v = std::getenv(name);
     if (!v) return dflt;
-    return std::strcmp(v, "runs1w3s") == 0  ? 9
-           : std::strcmp(v, "runs2sbc") == 0 ? 10
+    return std::strcmp(v, "runs2sbc") == 0  ? 10
            : std::strcmp(v, "runs2rs") == 0 ? 13
-           : std::strcmp(v, "runs1w2sbc") == 0 ? 11
-           : std::strcmp(v, "runs1w3sbc") == 0 ? 12
            : std::strcmp(v, "runs2d") == 0 ? 14
-           : std::strcmp(v, "runs1w2s") == 0 ? 8
-           : std::strcmp(v, "runs2st") == 0 ? 7
-           : std::strcmp(v, "runs2s") == 0 ? 6
-           : std::strcmp(v, "runs5") == 0 ? 5
-           : std::strcmp(v, "runs") == 0 ? 4
            : std::strcmp(v, "rows") == 0 ? 3
            : std::strcmp(v, "priv") == 0 ? 2
            : std::strcmp(v, "wave") == 0 ? 1
@@ -362,28 +354,66 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   // double: wavefronts sharing a tile with ds_add_f64
   spread_variant = variant("NFFT_HIP_SPREAD", sizeof(R) == 4 ? (d == 3 ? 10 : (d == 2 ? 14 : 2)) : 1);
 
-  // FFT: twiddles per dimension, then the pruned passes of both directions
+  // FFT: twiddles per dimension, then the pruned passes of both directions.
+  // An axis whose lines fit one workgroup's LDS (cap elements) and whose
+  // length factors into the stage radices is one pass; a longer one is
+  // split into two four-step halves n = n1 n2 (split[t] = n1), each fitting.
   const int cap = sizeof(R) == 4 ? 4096 : 2048;  // elements per workgroup
-  for (int t = 0; t < d; t++) {
-    FftAxis<R> ax{};
-    ax.len = (int)n[t];
-    if (n[t] > cap)
-      family<R>::fatal("nfft: FFT length n[%d] = %ld exceeds the supported %d", t, n[t], cap);
-    std::vector<int> rad;
-    if (!fft_radices(n[t], rad))
-      family<R>::fatal("nfft: FFT length n[%d] = %ld has a prime factor above 13", t, n[t]);
-    ax.nstages = (int)rad.size();
-    for (size_t i = 0; i < rad.size(); i++) ax.radix[i] = rad[i];
-    std::vector<cplx<R>> tw(n[t]);
-    for (long k = 0; k < n[t]; k++) {
-      long double a = -2.0L * kPiL * (long double)k / (long double)n[t];
+  auto twiddle_table = [&](long len) {
+    std::vector<cplx<R>> tw(len);
+    for (long k = 0; k < len; k++) {
+      long double a = -2.0L * kPiL * (long double)k / (long double)len;
       tw[k] = cplx<R>{(R)cosl(a), (R)sinl(a)};
     }
-    NFFT_HIP_CHECK(R, hipMalloc(&ax.twiddle, sizeof(cplx<R>) * tw.size()));
-    NFFT_HIP_CHECK(R, hipMemcpy(ax.twiddle, tw.data(), sizeof(cplx<R>) * tw.size(),
-                                hipMemcpyHostToDevice));
-    axes.push_back(ax);
+    cplx<R> *dev = nullptr;
+    NFFT_HIP_CHECK(R, hipMalloc(&dev, sizeof(cplx<R>) * tw.size()));
+    NFFT_HIP_CHECK(R, hipMemcpy(dev, tw.data(), sizeof(cplx<R>) * tw.size(), hipMemcpyHostToDevice));
+    return dev;
+  };
+  auto make_axis = [&](long len) {
+    FftAxis<R> ax{};
+    ax.len = (int)len;
+    std::vector<int> rad;
+    fft_radices(len, rad);
+    ax.nstages = (int)rad.size();
+    for (size_t i = 0; i < rad.size(); i++) ax.radix[i] = rad[i];
+    ax.twiddle = twiddle_table(len);
+    return ax;
+  };
+  long split[kMaxDim] = {0, 0, 0, 0};
+  std::vector<FftAxis<R>> sub1(d), sub2(d);
+  std::vector<cplx<R> *> tw_full(d, nullptr);
+  for (int t = 0; t < d; t++) {
+    std::vector<int> rad;
+    if (n[t] <= cap && fft_radices(n[t], rad)) {
+      axes.push_back(make_axis(n[t]));
+      continue;
+    }
+    // the most balanced n1 n2 = n with both halves fitting and factoring
+    long best = 0;
+    for (long a = 2; a <= cap && a < n[t]; a++) {
+      if (n[t] % a) continue;
+      const long b = n[t] / a;
+      std::vector<int> ra, rb;
+      if (b > cap || !fft_radices(a, ra) || !fft_radices(b, rb)) continue;
+      if (!best || std::labs(a - b) < std::labs(best - n[t] / best)) best = a;
+    }
+    if (!best)
+      family<R>::fatal("nfft: FFT length n[%d] = %ld is neither at most %d with prime factors "
+                       "up to 13 nor a product of two such lengths", t, n[t], cap);
+    split[t] = best;
+    axes.push_back(FftAxis<R>{});  // placeholder, the halves carry the tables
+    axes.back().len = (int)n[t];
+    sub1[t] = make_axis(n[t] / best);  // first half: length n2
+    sub2[t] = make_axis(best);         // second half: length n1
+    tw_full[t] = twiddle_table(n[t]);
+    extra_twiddles.push_back(sub1[t].twiddle);
+    extra_twiddles.push_back(sub2[t].twiddle);
+    extra_twiddles.push_back(tw_full[t]);
   }
+  bool four_step = false;
+  for (int t = 0; t < d; t++) four_step = four_step || split[t] > 0;
+  if (four_step) NFFT_HIP_CHECK(R, hipMalloc(&g2, sizeof(cplx<R>) * (size_t)n_total));
   {
     long gstride[kMaxDim], fstride[kMaxDim];
     int phioff[kMaxDim];
@@ -427,20 +457,66 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
       }
       p.fstr_ax = fstride[t];
       p.phi_ax = phi_inv + phioff[t];
+      const FftAxis<R> &ax = axes[t];
+      p.nsub = 1;
+      p.nfull = (int)n[t];
+      p.fn1 = p.fn2 = 1;
       // lines per workgroup that fit the LDS budget (the launch may use
       // fewer, see launch_fft_pass)
       p.lines_per_block = (int)std::max<long>(1, std::min<long>(256, cap / n[t]));
-      p.nstages = axes[t].nstages;
-      for (int i = 0; i < p.nstages; i++) p.radix[i] = axes[t].radix[i];
-      p.twiddle = axes[t].twiddle;
+      p.nstages = ax.nstages;
+      for (int i = 0; i < p.nstages; i++) p.radix[i] = ax.radix[i];
+      p.twiddle = ax.twiddle;
       const size_t line_bytes = sizeof(cplx<R>) * (size_t)p.lines_per_block * lds_line_stride(p.len);
       p.twiddle_in_lds = line_bytes + sizeof(cplx<R>) * (size_t)p.len <= 64 * 1024 ? 1 : 0;
       return p;
     };
+    // a four-step axis: its two halves in place of one pass.  The first
+    // keeps the pass's input rule and writes every position; the second
+    // reads every position and keeps the output rule.  cur = the buffer
+    // holding the data; the second half writes the other one.
+    auto halves = [&](FftPass<R> p, int t, int &cur, std::vector<FftPass<R>> &out) {
+      if (!split[t]) {
+        p.src = p.dst = cur;
+        out.push_back(p);
+        return;
+      }
+      const long n1 = split[t], n2 = n[t] / split[t];
+      FftPass<R> a = p, b = p;
+      for (int h = 0; h < 2; h++) {
+        FftPass<R> &q = h == 0 ? a : b;
+        const FftAxis<R> &ax = h == 0 ? sub1[t] : sub2[t];
+        q.four = h + 1;
+        q.fn1 = (int)n1;
+        q.fn2 = (int)n2;
+        q.nfull = (int)n[t];
+        q.twiddle_full = tw_full[t];
+        q.len = ax.len;
+        q.nsub = (int)(h == 0 ? n1 : n2);
+        q.nlines = p.nlines * q.nsub;
+        q.nstages = ax.nstages;
+        for (int i = 0; i < q.nstages; i++) q.radix[i] = ax.radix[i];
+        q.twiddle = ax.twiddle;
+        q.lines_per_block = (int)std::max<long>(1, std::min<long>(256, cap / q.len));
+        const size_t lb = sizeof(cplx<R>) * (size_t)q.lines_per_block * lds_line_stride(q.len);
+        q.twiddle_in_lds = lb + sizeof(cplx<R>) * (size_t)q.len <= 64 * 1024 ? 1 : 0;
+      }
+      a.out_mode = kToGrid;
+      b.in_mode = kFromGrid;
+      a.src = a.dst = cur;
+      b.src = cur;
+      b.dst = 1 - cur;
+      cur = 1 - cur;
+      out.push_back(a);
+      out.push_back(b);
+    };
+    int cur = 0;
     for (int t = d - 1; t >= 0; t--)
-      fwd_passes.push_back(make_pass(t, t == d - 1 ? kFromCoeffs : kFromGridBand, kToGrid));
+      halves(make_pass(t, t == d - 1 ? kFromCoeffs : kFromGridBand, kToGrid), t, cur, fwd_passes);
+    trafo_grid = cur;
+    cur = 0;
     for (int t = 0; t < d; t++)
-      bwd_passes.push_back(make_pass(t, kFromGrid, t == d - 1 ? kToCoeffs : kToGridBand));
+      halves(make_pass(t, kFromGrid, t == d - 1 ? kToCoeffs : kToGridBand), t, cur, bwd_passes);
   }
 }
 
@@ -449,6 +525,10 @@ void Engine<R>::destroy() {
   if (stream) hipStreamSynchronize(stream);
   for (auto &ax : axes) hipFree(ax.twiddle);
   axes.clear();
+  for (auto *tw : extra_twiddles) hipFree(tw);
+  extra_twiddles.clear();
+  hipFree(g2);
+  g2 = nullptr;
   hipFree(g);
   hipFree(fhat);
   hipFree(f);
@@ -527,9 +607,9 @@ void Engine<R>::trafo(const cplx<R> *fhat_dev, cplx<R> *f_dev) {
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[0], stream));
   // D runs inside the first FFT pass: the D interval stays empty
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[1], stream));
-  for (auto &ps : fwd_passes) launch_fft_pass<R>(g, fhat_dev, nullptr, ps, -1, stream);
+  for (auto &ps : fwd_passes) launch_fft_pass<R>(g, fhat_dev, nullptr, ps, -1, stream, 1, g2);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[2], stream));
-  launch_interp<R>(g, f_dev, *this, stream);
+  launch_interp<R>(grid_after_trafo(), f_dev, *this, stream);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[3], stream));
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[4], stream));
   NFFT_HIP_CHECK(R, hipGetLastError());
@@ -548,7 +628,7 @@ void Engine<R>::adjoint(const cplx<R> *f_dev, cplx<R> *fhat_dev) {
   if (scratch) launch_gather_tiles<R>(*this, g, stream);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[2], stream));
   // D^T runs inside the last FFT pass: the D^T interval stays empty
-  for (auto &ps : bwd_passes) launch_fft_pass<R>(g, nullptr, fhat_dev, ps, +1, stream);
+  for (auto &ps : bwd_passes) launch_fft_pass<R>(g, nullptr, fhat_dev, ps, +1, stream, 1, g2);
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[3], stream));
   if (E) NFFT_HIP_CHECK(R, hipEventRecord(E[4], stream));
   NFFT_HIP_CHECK(R, hipGetLastError());
@@ -556,7 +636,9 @@ void Engine<R>::adjoint(const cplx<R> *f_dev, cplx<R> *fhat_dev) {
 
 template <typename R>
 bool Engine<R>::batchable() const {
-  return !direct_only && !timing && !deferred && M > 0 && sp_nsub > 0 && batch_kernels<R>(*this);
+  // four-step axes (g2) run their right-hand sides one by one
+  return !direct_only && !timing && !deferred && M > 0 && sp_nsub > 0 && !g2 &&
+         batch_kernels<R>(*this);
 }
 
 // right-hand sides per batched launch: the batch grids and tiles of one
@@ -622,7 +704,14 @@ void Engine<R>::adjoint_batch(const cplx<R> *f_dev, cplx<R> *fhat_dev, int nrhs)
     return;
   }
   reserve_batch(nrhs);
-  const int k = batch_chunk(nrhs);
+  adjoint_batch_chunks(f_dev, fhat_dev, nrhs, batch_chunk(nrhs), bgrid, btiles);
+}
+
+// batched adjoint in chunks of k right-hand sides on caller-given batch
+// grids (k n_total values) and tiles (k sp_nsub E^d values)
+template <typename R>
+void Engine<R>::adjoint_batch_chunks(const cplx<R> *f_dev, cplx<R> *fhat_dev, int nrhs, int k,
+                                     cplx<R> *grids, cplx<R> *tls) {
   long tot = 1;
   for (int t = 0; t < d; t++) tot *= geom.E[t];
   BinGeom gb = geom;
@@ -631,13 +720,14 @@ void Engine<R>::adjoint_batch(const cplx<R> *f_dev, cplx<R> *fhat_dev, int nrhs)
   gb.bs_t = sp_nsub * tot;
   for (int r0 = 0; r0 < nrhs; r0 += k) {
     const int nb = std::min(k, nrhs - r0);
-    launch_spread<R>(f_dev + (long)r0 * M, bgrid, *this, stream, nb, &gb, btiles);
-    launch_gather_tiles<R>(*this, bgrid, stream, nb, &gb, btiles);
+    launch_spread<R>(f_dev + (long)r0 * M, grids, *this, stream, nb, &gb, tls);
+    launch_gather_tiles<R>(*this, grids, stream, nb, &gb, tls);
     for (auto &ps : bwd_passes)
-      launch_fft_pass<R>(bgrid, nullptr, fhat_dev + (long)r0 * N_total, ps, +1, stream, nb);
+      launch_fft_pass<R>(grids, nullptr, fhat_dev + (long)r0 * N_total, ps, +1, stream, nb);
   }
   NFFT_HIP_CHECK(R, hipGetLastError());
 }
+
 
 template <typename R>
 void Engine<R>::autotune() {
@@ -661,7 +751,7 @@ void Engine<R>::autotune() {
     // float candidates accumulate in plain FP32 (the fixed-point LDS
     // variants are left out); double ones use ds_add_f64
     std::vector<int> cand;
-    if (sizeof(R) == 4 && d == 3) cand = {10, 13, 11, 6, 8, 9, 7, 4, 5, 3};
+    if (sizeof(R) == 4 && d == 3) cand = {10, 13, 3};
     else if (sizeof(R) == 4 && d == 2) cand = {14, 1, 0, 2};
     else if (sizeof(R) == 8 && d == 2) cand = {14, 1, 0};
     else if (sizeof(R) == 8 && d >= 2) cand = {1, 0};

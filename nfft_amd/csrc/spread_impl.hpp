@@ -1468,17 +1468,12 @@ template <typename R> struct BKernelTable {
   fn spread_wave[kMaxDim][kMaxStaticM + 1];
   fn spread_priv[kMaxDim][kMaxStaticM + 1];  // d = 2, 3 only, 64 threads
   fn spread_rows[kMaxStaticM + 1];            // d = 3 only, 128 threads
-  fn spread_runs[kMaxStaticM + 1];            // d = 3, float only, 128 threads
-  fn spread_runs_wide[kMaxStaticM + 1];       // same, runs_wide_nw(W) wavefronts
-  fn spread_runs_2s[kMaxStaticM + 1];         // same, two node streams per wavefront
-  fn spread_runs_2st[kMaxStaticM + 1];        // same, row weights through LDS
-  fn spread_runs_2sbc[kMaxStaticM + 1];       // same, row weights by DPP row broadcast
-  fn spread_runs_2rs[kMaxStaticM + 1];        // two wavefronts, one stream, run load-store
-  fn spread_runs_1w2sbc[kMaxStaticM + 1];     // one wavefront per tile, two streams, DPP
+  // d = 3, float only, 128 threads: run merging, two wavefronts per tile,
+  // row weights by DPP row broadcast; two node streams per wavefront (2sbc)
+  // or one stream whose runs load and store their tile cells (2rs)
+  fn spread_runs_2sbc[kMaxStaticM + 1];
+  fn spread_runs_2rs[kMaxStaticM + 1];
   fn spread_runs2d[kMaxStaticM + 1];          // d = 2: run-merging, one wavefront
-  fn spread_runs_1w3sbc[kMaxStaticM + 1];     // one wavefront per tile, three streams, DPP
-  fn spread_runs_1w2s[kMaxStaticM + 1];       // one wavefront per tile, two streams
-  fn spread_runs_1w3s[kMaxStaticM + 1];       // one wavefront per tile, three streams
   fn interp_rt[kMaxDim][2];  // [d-1][tile]
   fn spread_rt[kMaxDim][2];
 };
@@ -1504,19 +1499,10 @@ void bk_fill_one(BKernelTable<R> &k) {
                                                : ((2 * M_ + 2) * (M_ + 1) > 64 ? 2 : 4);
     k.spread_runs2d[M_] = k_spread_runs2d<R, 2 * M_ + 2, G2, 2>;
   }
-  if constexpr (std::is_same_v<R, float> && D == 3 && D * (2 * M_ + 2) <= 64) {
-    k.spread_runs[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 1, false>;
-    k.spread_runs_wide[M_] = k_spread_runs<2 * M_ + 2, runs_wide_nw(2 * M_ + 2), 8, 1, false>;
-    k.spread_runs_2s[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false>;
-    k.spread_runs_2st[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, true>;
-    k.spread_runs_2sbc[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false, true>;
-    k.spread_runs_2rs[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 1, false, true, true>;
-    if constexpr (2 * M_ + 2 <= 16) {  // all rows' weights in one 16-lane DPP row
-      k.spread_runs_1w2sbc[M_] = k_spread_runs<2 * M_ + 2, 1, 8, 2, false, true>;
-      k.spread_runs_1w3sbc[M_] = k_spread_runs<2 * M_ + 2, 1, 8, 3, false, true>;
-    }
-    k.spread_runs_1w2s[M_] = k_spread_runs<2 * M_ + 2, 1, 8, 2, false>;
-    k.spread_runs_1w3s[M_] = k_spread_runs<2 * M_ + 2, 1, 8, 3, false>;
+  // (the row weights of a wavefront's W/2 rows fit one 16-lane DPP row)
+  if constexpr (std::is_same_v<R, float> && D == 3 && D * (2 * M_ + 2) <= 64 && M_ + 1 <= 16) {
+    k.spread_runs_2sbc[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2>;
+    k.spread_runs_2rs[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 1, true>;
   }
 }
 
@@ -1568,17 +1554,9 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
   threads = 256;
   if (tile && gg.d <= 3 && gg.m >= 1 && gg.m <= kMaxStaticM) {
     const int t = gg.d - 1;
-    if (spread && (variant == 8 || variant == 9) && gg.d == 3 && k.spread_runs_1w2s[gg.m]) {
-      threads = 64;
-      return variant == 8 ? k.spread_runs_1w2s[gg.m] : k.spread_runs_1w3s[gg.m];
-    }
     if (spread && variant == 14 && gg.d == 2 && k.spread_runs2d[gg.m]) {
       threads = 64;
       return k.spread_runs2d[gg.m];
-    }
-    if (spread && (variant == 11 || variant == 12) && gg.d == 3 && k.spread_runs_1w2sbc[gg.m]) {
-      threads = 64;
-      return variant == 11 ? k.spread_runs_1w2sbc[gg.m] : k.spread_runs_1w3sbc[gg.m];
     }
     if (spread && variant == 10 && gg.d == 3 && k.spread_runs_2sbc[gg.m]) {
       threads = 128;
@@ -1587,24 +1565,6 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
     if (spread && variant == 13 && gg.d == 3 && k.spread_runs_2rs[gg.m]) {
       threads = 128;
       return k.spread_runs_2rs[gg.m];
-    }
-    if (spread && variant == 7 && gg.d == 3 && k.spread_runs_2st[gg.m]) {
-      threads = 128;
-      // row-weight table: [2 wavefronts][2 streams][8 nodes][W/2 rows]
-      lds = bytes + sizeof(cplx<R>) * 2 * 2 * 8 * (gg.m + 1);
-      return k.spread_runs_2st[gg.m];
-    }
-    if (spread && variant == 6 && gg.d == 3 && k.spread_runs_2s[gg.m]) {
-      threads = 128;
-      return k.spread_runs_2s[gg.m];
-    }
-    if (spread && variant == 5 && gg.d == 3 && k.spread_runs_wide[gg.m]) {
-      threads = 64 * runs_wide_nw(2 * gg.m + 2);
-      return k.spread_runs_wide[gg.m];
-    }
-    if (spread && variant >= 4 && gg.d == 3 && k.spread_runs[gg.m]) {
-      threads = 128;
-      return k.spread_runs[gg.m];
     }
     if (spread && variant >= 3 && gg.d == 3 && k.spread_rows[gg.m]) {
       threads = 128;

@@ -18,8 +18,9 @@
 // wt, tap ww) of one node as a per-tap polynomial (two nodes per pass when
 // 3 W <= 32), with the node's fractional coordinates and value fetched by
 // ds_bpermute from the lane that loaded the node.  A lane's pair weights
-// psi1 * psi2 are fetched the same way; the row weights psi0 * f_j are
-// broadcast with readlane.
+// psi1 * psi2 are fetched the same way; the row weights psi0 * f_j of each
+// node are gathered once by ds_bpermute into lanes 0..RW-1 of every 16-lane
+// row and the FMAs read them with a DPP row broadcast.
 #pragma once
 
 #include "engine.hpp"
@@ -29,10 +30,6 @@ namespace nfft_hip {
 
 __device__ __forceinline__ float bperm_f(int byte_addr, float v) {
   return __int_as_float(__builtin_amdgcn_ds_bpermute(byte_addr, __float_as_int(v)));
-}
-
-__device__ __forceinline__ float rlane_f(float v, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
 // acc += x[lane J of this lane's 16-lane row] * w: the row-broadcast DPP
@@ -47,6 +44,9 @@ __device__ __forceinline__ void fmac_row_bcast(float &acc, float x, float w) {
 
 // The RW rows of one node: acc[j] += (sr, si)[row j] * (w0, w1) for the
 // lane's two pair cells, the row weights taken from lanes j of every row.
+// (A run's first node setting the accumulators with v_mul_f32_dpp instead of
+// clearing them at each flush put the FMAs of every node behind a branch and
+// measured slower: B^T 4.41 -> 4.83 ms on the 3-D kooshball.)
 template <int J, int RW, int SLOTS>
 __device__ __forceinline__ void rows_bcast(float2 (&acc)[RW][SLOTS][2], float br, float bi,
                                            const float (&w0)[SLOTS], const float (&w1)[SLOTS]) {
@@ -60,30 +60,17 @@ __device__ __forceinline__ void rows_bcast(float2 (&acc)[RW][SLOTS][2], float br
   if constexpr (J + 1 < RW) rows_bcast<J + 1, RW, SLOTS>(acc, br, bi, w0, w1);
 }
 
-// More wavefronts per tile hide more latency but repeat the per-node window
-// work in each: the widest split with at most five rows groups, NW | W.
-constexpr int runs_wide_nw(int W) {
-  return W % 5 == 0 ? 5 : (W % 4 == 0 ? 4 : (W % 3 == 0 ? 3 : 2));
-}
-
 // S node streams per wavefront: stream s walks its own part of the work
 // item's nodes and the updates of the streams interleave node by node, which
 // gives the wavefront S independent dependency chains.  Streams of one
 // wavefront may touch the same tile cells; a wavefront's LDS operations
 // complete in program order, so their read-modify-writes do not race.
-// TAB: the row weights psi0 * f_j of this wavefront's rows go through a small
-// LDS table (written by the lanes that evaluated them, read back as uniform
-// loads) instead of one readlane per row and component.
-// BC: the row weights of each node are gathered once by ds_bpermute into
-// lanes 0..RW-1 of every 16-lane row and the FMAs read them with a DPP row
-// broadcast, replacing 2 RW readlanes per node (and their scalar index
-// arithmetic) by two ds_bpermute.
 // RS (one stream): a run loads its cells of the tile into the accumulators
 // when it starts and stores them back when it ends, with no add and no
 // clearing; no other stream of the wavefront writes those cells meanwhile
 // and the other wavefronts own other rows, so the load-store pair is
 // race-free.
-template <int W, int NW, int G, int S, bool TAB, bool BC = false, bool RS = false>
+template <int W, int NW, int G, int S, bool RS = false>
 __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restrict__ fin,
                                                          float2 *__restrict__ g,
                                                          const int *__restrict__ perm,
@@ -100,7 +87,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
   constexpr int NP = 3 * W <= 32 ? 2 : 1;  // nodes per window pass
   constexpr int NPASS = G / NP;
   static_assert(G % NP == 0 && 64 % G == 0, "groups tile the 64-node chunk");
-  static_assert(!(BC && TAB) && (!BC || RW <= 16), "one row-weight path; RW rows fit a DPP row");
+  static_assert(RW <= 16, "a wavefront's rows fit one 16-lane DPP row");
   static_assert(!RS || S == 1, "run load-store needs one stream per wavefront");
   // right-hand side blockIdx.y of a batched launch
   fin += (long)blockIdx.y * gg.bs_f;
@@ -159,8 +146,6 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
       sn[st] = min(sp.z, (st + 1) * part) - sb[st];
     }
   }
-  // row-weight table after the tile: [wavefront][stream][node of group][row]
-  float2 *rtab = tile + tot + (long)wv * S * G * RW;
   // run accumulators per stream: [row j][slot][pair cell], complex
   float2 acc[S][RW][SLOTS][2];
 #pragma unroll
@@ -174,6 +159,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
   // stream's first node on (a stream without nodes holds the item's first
   // node's corner and only ever adds zeros there)
   int cur[S];
+  // the run's sums into the tile (read-modify-write), accumulators cleared
   auto flush = [&](int st) {
     const int corner = cur[st] & 0xfffff;
     const int r0 = cur[st] >> 22;
@@ -315,13 +301,6 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
           mine[p] = v;
           xr[st][p] = v * fx;
           xi[st][p] = v * fy;
-          if constexpr (TAB) {
-            // lanes holding a row tap of this wavefront's rows store its weight
-            const int r0n = __builtin_amdgcn_ds_bpermute(src, ckey[st]) >> 22;
-            const int d = ww - r0n;
-            if (wt == 0 && li < W && d >= 0 && d % NW == 0)
-              rtab[(st * G + NP * p + h) * RW + d / NW] = float2{xr[st][p], xi[st][p]};
-          }
         }
 #pragma unroll
         for (int k = 0; k < G; k++) {
@@ -334,11 +313,11 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
           }
         }
       }
-      // BC: keys of the group's nodes and their row weights, lane q of every
+      // keys of the group's nodes and their row weights, lane q of every
       // 16-lane row holding row r0 + q NW of the node
       int gkey[S][G];
       float brr[S][G], bri[S][G];
-      if constexpr (BC) {
+      {
         const int qoff = 4 * NW * (lane & 15);
 #pragma unroll
         for (int st = 0; st < S; st++)
@@ -355,54 +334,24 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
       for (int k = 0; k < G; k++) {
 #pragma unroll
         for (int st = 0; st < S; st++) {
-          if constexpr (BC) {
-            const int key = gkey[st][k];
-            float a0[SLOTS], a1[SLOTS];
+          const int key = gkey[st][k];
+          float a0[SLOTS], a1[SLOTS];
 #pragma unroll
-            for (int sl = 0; sl < SLOTS; sl++) {
-              a0[sl] = w0[st][k][sl];
-              a1[sl] = w1[st][k][sl];
-            }
-            if (key != cur[st]) {
-              if constexpr (RS) {
-                store(st);
-                cur[st] = key;
-                load(st);
-              } else {
-                flush(st);
-                cur[st] = key;
-              }
-            }
-            rows_bcast<0, RW, SLOTS>(acc[st], brr[st][k], bri[st][k], a0, a1);
-            continue;
+          for (int sl = 0; sl < SLOTS; sl++) {
+            a0[sl] = w0[st][k][sl];
+            a1[sl] = w1[st][k][sl];
           }
-          {
-            const int p = k / NP, hk = k % NP;
-            const int key = __builtin_amdgcn_readlane(ckey[st], base + k);
-            if (key != cur[st]) {
+          if (key != cur[st]) {
+            if constexpr (RS) {
+              store(st);
+              cur[st] = key;
+              load(st);
+            } else {
               flush(st);
               cur[st] = key;
             }
-            const int r0 = key >> 22;
-            float2 rw[RW];
-            if constexpr (TAB) {
-#pragma unroll
-              for (int j = 0; j < RW; j++) rw[j] = rtab[(st * G + k) * RW + j];
-            }
-#pragma unroll
-            for (int j = 0; j < RW; j++) {
-              const int src = hk * 32 + r0 + j * NW;
-              const float sr = TAB ? rw[j].x : rlane_f(xr[st][p], src);
-              const float si = TAB ? rw[j].y : rlane_f(xi[st][p], src);
-#pragma unroll
-              for (int sl = 0; sl < SLOTS; sl++) {
-                acc[st][j][sl][0].x = fmaf(w0[st][k][sl], sr, acc[st][j][sl][0].x);
-                acc[st][j][sl][0].y = fmaf(w0[st][k][sl], si, acc[st][j][sl][0].y);
-                acc[st][j][sl][1].x = fmaf(w1[st][k][sl], sr, acc[st][j][sl][1].x);
-                acc[st][j][sl][1].y = fmaf(w1[st][k][sl], si, acc[st][j][sl][1].y);
-              }
-            }
           }
+          rows_bcast<0, RW, SLOTS>(acc[st], brr[st][k], bri[st][k], a0, a1);
         }
       }
     }

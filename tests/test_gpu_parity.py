@@ -93,7 +93,7 @@ def test_vs_oracle(case, precision):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant", ["lane", "wave", "priv", "rows", "runs", "runs5", "runs2s", "runs2st", "runs1w2s", "runs1w3s", "runs2sbc", "runs1w2sbc", "runs1w3sbc", "runs2rs", "runs2d"])
+@pytest.mark.parametrize("variant", ["lane", "wave", "priv", "rows", "runs2sbc", "runs2rs", "runs2d"])
 @pytest.mark.parametrize("precision", ["double", "single"])
 @pytest.mark.parametrize("case", [c for c in CASES if len(c[1]) >= 2], ids=[c[0] for c in CASES if len(c[1]) >= 2])
 def test_b_kernel_variants(case, precision, variant, monkeypatch):

