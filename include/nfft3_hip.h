@@ -66,6 +66,20 @@ typedef struct nfft_hip_graph_s nfft_hip_graph;
                              int nrhs);                                      \
   void P##adjoint_batch_device(PLAN *ths, const void *f_dev,                \
                                void *f_hat_dev, int nrhs);                   \
+  /* adjoint_batch_device that allocates nothing.  No counterpart in the    \
+   * reference; for callers that cannot allocate device memory (code under  \
+   * HIP graph capture, tight serving loops).  The batch grids and spreading \
+   * tiles live in one static 64 MiB device array per GPU, held until the    \
+   * process ends, which each chunk of right-hand sides reuses; a right-hand \
+   * side needing more than 64 MiB (grid plus tiles) is refused.  The plan   \
+   * needs a PRE_* psi flag with precompute_one_psi done and stage timing    \
+   * off.  Returns 0 when the work is enqueued, else a negative code (-1     \
+   * nrhs < 0, -2 plan not ready, -3 scratch address unknown, -4 too large)  \
+   * with a message on stderr, and enqueues nothing.  Calls on one GPU share \
+   * the array: calls that overlap in time on one GPU (other streams, other  \
+   * threads) race and must be serialised by the caller. */                 \
+  int P##adjoint_batch_device_noalloc(PLAN *ths, const void *f_dev,         \
+                                      void *f_hat_dev, int nrhs);            \
   /* Capture the batched transforms of count plans (nrhs each, device       \
    * arrays f_hat_dev[i] / f_dev[i] of plan i) into one hipGraph.  Plans on  \
    * different streams run concurrently on replay; the graph replays on     \

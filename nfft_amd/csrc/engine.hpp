@@ -247,6 +247,14 @@ template <typename R> struct Engine {
   void adjoint_batch(const cplx<R> *f_dev, cplx<R> *fhat_dev, int nrhs);
   void adjoint_batch_chunks(const cplx<R> *f_dev, cplx<R> *fhat_dev, int nrhs, int k,
                             cplx<R> *grids, cplx<R> *tls);
+  // adjoint_batch with its batch grids and tiles in the library's static
+  // per-GPU scratch array (plan.hip) instead of plan allocations: 0 when the
+  // work is enqueued, -3 when the scratch address is unknown, -4 when one
+  // right-hand side needs more than the array holds
+  int device = 0;
+  void *noalloc_scratch = nullptr;  // device address of the scratch array
+  long noalloc_bytes_per_rhs() const;
+  int adjoint_batch_noalloc(const cplx<R> *f_dev, cplx<R> *fhat_dev, int nrhs);
   cplx<R> *bgrid = nullptr;   // batched grids
   cplx<R> *btiles = nullptr;  // batched spreading tiles
   long bgrid_cap = 0, btiles_cap = 0;  // in complex values

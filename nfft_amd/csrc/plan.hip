@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/nfft3.h"
 #include "../../include/nfft3_hip.h"
@@ -28,6 +29,37 @@ struct nfft_hip_graph_s {
 };
 
 namespace nfft_hip {
+
+// ------------------------------------------------------------------ scratch
+// Device memory of the allocation-free batched adjoint
+// (nfft[f]_adjoint_batch_device_noalloc): the batch grids and spreading
+// tiles of one chunk of right-hand sides.  One copy per GPU the library runs
+// on, held until the process ends.  On the host the array's name is only a
+// stand-in for the symbol; its device address on each GPU comes from
+// hipGetSymbolAddress, looked up once per device when the first plan is
+// created there, and only that address is handed to kernels.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[kNoAllocMaxDevices];
+
+// device address of `scratch` on device dev (the current device), looked up
+// on the first call; nullptr when HIP does not provide it
+static void *scratch_setup(int dev) {
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    g_scratch_dev[dev] = ptr;
+  }
+  return g_scratch_dev[dev];
+}
 
 // ------------------------------------------------------------------ helpers
 // I0 by its power series sum_k ((x/2)^2)^k / (k!)^2, in long double.
@@ -145,6 +177,8 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     family<R>::fatal("nfft: no HIP device available (this library runs on the GPU only)");
+  NFFT_HIP_CHECK(R, hipGetDevice(&device));
+  noalloc_scratch = scratch_setup(device);
   NFFT_HIP_CHECK(R, hipStreamCreate(&own_stream));
   stream = own_stream;
   for (auto &e : ev) NFFT_HIP_CHECK(R, hipEventCreate(&e));
@@ -728,6 +762,34 @@ void Engine<R>::adjoint_batch_chunks(const cplx<R> *f_dev, cplx<R> *fhat_dev, in
   NFFT_HIP_CHECK(R, hipGetLastError());
 }
 
+// Bytes of `scratch` that one right-hand side of a batched adjoint needs:
+// its grid and its spreading tiles, each rounded up to 256 bytes.
+template <typename R>
+long Engine<R>::noalloc_bytes_per_rhs() const {
+  long tot = 1;
+  for (int t = 0; t < d; t++) tot *= geom.E[t];
+  auto up = [](long b) { return (b + 255) / 256 * 256; };
+  return up(n_total * (long)sizeof(cplx<R>)) + up(sp_nsub * tot * (long)sizeof(cplx<R>));
+}
+
+template <typename R>
+int Engine<R>::adjoint_batch_noalloc(const cplx<R> *f_dev, cplx<R> *fhat_dev, int nrhs) {
+  if (nrhs < 2 || !batchable()) {
+    // one by one on the plan's own grid and tiles
+    for (int r = 0; r < nrhs; r++) adjoint(f_dev + (long)r * M, fhat_dev + (long)r * N_total);
+    return 0;
+  }
+  if (!noalloc_scratch) return -3;
+  const long per = noalloc_bytes_per_rhs();
+  if (per > (long)kNoAllocScratchBytes) return -4;
+  const int k = (int)std::min<long>(nrhs, (long)kNoAllocScratchBytes / per);
+  // k grids, then k tile sets, each block 256-byte aligned
+  char *base = static_cast<char *>(noalloc_scratch);
+  const long grid_bytes = ((long)k * n_total * (long)sizeof(cplx<R>) + 255) / 256 * 256;
+  adjoint_batch_chunks(f_dev, fhat_dev, nrhs, k, reinterpret_cast<cplx<R> *>(base),
+                       reinterpret_cast<cplx<R> *>(base + grid_bytes));
+  return 0;
+}
 
 template <typename R>
 void Engine<R>::autotune() {
@@ -1264,6 +1326,46 @@ struct PlanApi {
     e->adjoint_batch((const cplx<R> *)f, (cplx<R> *)fh, nrhs);
   }
 
+  // adjoint_batch_device without any allocation (nfft3_hip.h): the batch
+  // grids and tiles of each chunk of right-hand sides live in the library's
+  // static scratch array, so the plan must have its node data already (a
+  // PRE_* psi flag and precompute_one_psi called) and stage timing off.
+  // Returns 0 when the work is enqueued, else a negative code with a message
+  // on stderr and nothing enqueued.
+  static int adjoint_batch_device_noalloc(PLAN *p, const void *f, void *fh, int nrhs) {
+    Engine<R> *e = eng(p);
+    const char *why = nullptr;
+    int rc = 0;
+    if (nrhs < 0) {
+      rc = -1;
+      why = "nrhs < 0";
+    } else if (e->timing || e->deferred) {
+      rc = -2;
+      why = "stage timing is on (it records into events it creates)";
+    } else if (!e->direct_only && (!(p->flags & PRE_ONE_PSI) || !e->nodes_ready)) {
+      rc = -2;
+      why = "node data not built: use a PRE_* psi flag and call precompute_one_psi first";
+    }
+    if (!rc && nrhs > 0) {
+      clear_launch_error();
+      if (e->direct_only) {
+        e->upload_nodes(p->x);  // the NDFT reads the current x (nfft.c:145-297)
+        for (int r = 0; r < nrhs; r++)
+          e->adjoint_direct((const cplx<R> *)f + (long)r * p->M_total,
+                            (cplx<R> *)fh + (long)r * p->N_total);
+      } else {
+        rc = e->adjoint_batch_noalloc((const cplx<R> *)f, (cplx<R> *)fh, nrhs);
+        if (rc == -3) why = "the scratch array's device address is unknown on this device";
+        if (rc == -4) why = "one right-hand side needs more than the 64 MiB scratch array";
+      }
+    }
+    if (rc)
+      std::fprintf(stderr, "%sadjoint_batch_device_noalloc: %s (per right-hand side %ld bytes)\n",
+                   sizeof(R) == 4 ? "nfftf_" : "nfft_", why,
+                   e->direct_only ? 0L : e->noalloc_bytes_per_rhs());
+    return rc;
+  }
+
   // One hipGraph for the batched transforms of `count` plans: captured on
   // plans[0]'s stream, each other plan's stream forks from it and joins it
   // again, so plans on different streams run concurrently on replay.  Node
@@ -1448,6 +1550,9 @@ using nfft_hip::PlanApi;
   }                                                                                           \
   void P##adjoint_batch_device(PLAN *p, const void *f, void *fh, int nrhs) {                  \
     PlanApi<R, PLAN>::adjoint_batch_device(p, f, fh, nrhs);                                   \
+  }                                                                                           \
+  int P##adjoint_batch_device_noalloc(PLAN *p, const void *f, void *fh, int nrhs) {           \
+    return PlanApi<R, PLAN>::adjoint_batch_device_noalloc(p, f, fh, nrhs);                    \
   }                                                                                           \
   nfft_hip_graph *P##graph_trafo(PLAN **plans, int count, const void **fh, void **f,          \
                                  int nrhs) {                                                  \
