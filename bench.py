@@ -101,23 +101,29 @@ def oracle_sample_errors(N, n, m, x, fh, f, t64, a64, sample, seed):
 
 
 def cpu_baseline(N, n, m, x, fh, f, sample, threads):
-    """FP64 oracle on the host (its FFT and B / B^T loops on `threads` OpenMP
-    threads): full-grid D + FFT both ways, B and B^T on `sample` nodes, B
-    time scaled to all nodes."""
+    """FP64 CPU path on the host's cores (OpenMP, `threads` threads), shaped as
+    the reference's PRE_PSI trafo / blockwise adjoint (oracle.Prepsi3d: window
+    values precomputed per node and not timed, as nfft_benchomp times
+    precompute separately; tensor loops; the adjoint split into grid slabs
+    without atomics), with the oracle's full-grid D + FFT both ways; B and
+    B^T on `sample` nodes, their time scaled to all nodes."""
     import oracle as O
     O.set_threads(threads)
     M = x.shape[0]
     idx = np.random.default_rng(5).choice(M, size=sample, replace=False)
     xs = np.ascontiguousarray(x[idx])
+    tp = time.perf_counter()
+    P = O.Prepsi3d(N, n, m, xs)
+    t_pre = time.perf_counter() - tp
     t0 = time.perf_counter()
     g = O.stage_D(N, n, m, fh)
     g = O.fft(n, g, -1)
     t1 = time.perf_counter()
-    O.stage_B(N, n, m, xs, g)
+    P.B(g)
     t2 = time.perf_counter()
-    gt = O.stage_BT(N, n, m, xs, f[idx])
+    gt = P.BT(f[idx])
     t3 = time.perf_counter()
-    gt = O.fft(n, g, +1)
+    gt = O.fft(n, gt, +1)
     O.stage_DT(N, n, m, gt)
     t4 = time.perf_counter()
     O.set_threads(1)
@@ -125,10 +131,12 @@ def cpu_baseline(N, n, m, x, fh, f, sample, threads):
     per_node = ((t2 - t1) + (t3 - t2)) / sample
     est = fixed + per_node * M
     return dict(value=M / est, unit="nodes/s", cores=threads, kind="port",
-                sample="oracle (FP64, %d OpenMP threads; the reference itself needs FFTW, absent "
-                       "here) on the same workload: full 256^3 D+FFT for trafo and adjoint, B and "
-                       "B^T on %d random nodes of the kooshball, B time scaled to M=%d; measured "
-                       "%.1f s wall" % (threads, sample, M, t4 - t0))
+                sample="FP64 CPU path shaped as the reference's PRE_PSI trafo and blockwise adjoint "
+                       "(oracle.Prepsi3d; the reference itself needs FFTW, absent here) on %d OpenMP "
+                       "threads, same workload: full 256^3 D+FFT for trafo and adjoint, B and B^T on "
+                       "%d random nodes of the kooshball, B time scaled to M=%d; B %.2f s, B^T %.2f s, "
+                       "D+FFT both ways %.2f s, untimed window precompute %.2f s"
+                       % (threads, sample, M, t2 - t1, t3 - t2, fixed, t_pre))
 
 
 def cgnr_leg(args):
@@ -443,6 +451,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cgnr", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--estimate", action="store_true",
+                    help="plan the headline workload with FFTW_ESTIMATE (built-in kernel "
+                         "choices) instead of FFTW_MEASURE (timed at precompute)")
     ap.add_argument("--no-error", action="store_true",
                     help="skip the accuracy figures (FP64 plan and oracle sample)")
     ap.add_argument("--cgnr-max-iter", type=int, default=1000)
@@ -484,7 +495,12 @@ def main():
 
     flags = (nf.PRE_PHI_HUT | nf.PRE_PSI | nf.MALLOC_X | nf.MALLOC_F_HAT | nf.MALLOC_F
              | nf.FFTW_INIT | nf.NFFT_SORT_NODES)
-    p = nf.Plan(N, M, n=n, m=m, flags=flags, precision="single")
+    # FFTW_MEASURE: precompute times the bin edges, B / B^T kernel shapes,
+    # spreading item size and FFT lines per workgroup on these nodes and
+    # keeps the fastest (included in precompute_ms, outside the timed steps)
+    fftw_flags = nf.FFTW_ESTIMATE if args.estimate else nf.FFTW_MEASURE
+    p = nf.Plan(N, M, n=n, m=m, flags=flags, precision="single",
+                fftw_flags=fftw_flags | nf.FFTW_DESTROY_INPUT)
     p.x[:] = x
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
@@ -593,6 +609,7 @@ def main():
             "workload": "3-D NFFT trafo+adjoint, FP32, N=128^3, sigma=2 (n=256^3), Kaiser-Bessel m=4, "
                         "M=2^24 kooshball nodes (16384 spokes x 1024 samples), device-resident",
             "N": N, "n": n, "m": m, "M": M, "parallelism": "single GPU",
+            "planning": "FFTW_ESTIMATE" if args.estimate else "FFTW_MEASURE",
         },
         "roofline": {
             "bound": "hbm",

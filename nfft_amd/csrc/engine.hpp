@@ -42,9 +42,10 @@ struct BinGeom {
   // batched launches (blockIdx.y = right-hand side): per right-hand side
   // offsets of the grid, the node vector and the scratch tiles; 0 otherwise
   long bs_g, bs_f, bs_t;
-  // interpolation slots (see nodes.hip): slots[k] = (node, second node of
-  // the same cell or -1); islots[item] = (first slot, slot count); null when
-  // the plan's interpolation kernel does not use them
+  // interpolation slots (see nodes.hip): slots[k] = (first node, count) of
+  // up to slot_nodes consecutive nodes of one cell; islots[item] = (first
+  // slot, slot count); null when the plan's interpolation kernel does not
+  // use them
   const int2 *slots;
   const int2 *islots;
   // LDS row pitch (cells) of the 3-D lane-per-node interpolation tile:
@@ -137,6 +138,9 @@ template <typename R> struct FftPass {
   // grid buffer the pass reads and the one it writes: 0 the plan's grid, 1
   // its second grid (four-step plans only)
   int src, dst;
+  // lines per workgroup relative to the launch's default choice: times
+  // 2^lpb_shift (FFTW_MEASURE plans time -1, 0, +1)
+  int lpb_shift;
 };
 
 template <typename R> struct Engine {
@@ -192,6 +196,10 @@ template <typename R> struct Engine {
   // single-node kernel.
   int pair_interp = 1;
   bool use_slots() const { return sizeof(R) == 4 && d == 3 && pair_interp; }
+  // nodes per interpolation slot: 4 (quads where a cell has that many;
+  // wavefronts whose slots hold at most two run the pair code), or 2
+  // (NFFT_HIP_SLOT=2)
+  int slot_nodes = 4;
   // device scratch of the density-weight helpers (features, Pipe-Menon),
   // grown on demand and kept with the plan so repeated calls allocate nothing
   char *aux = nullptr;
@@ -225,12 +233,14 @@ template <typename R> struct Engine {
   cplx<R> *grid_after_trafo() const { return trafo_grid ? g2 : g; }
 
   void create(int d_, const long *N_, const long *n_, long M_, int m_);
+  void set_bins(const int *bs);
   void destroy();
   void upload_nodes(const R *x_host);
   void build_nodes();  // from xdev
-  // FFTW_MEASURE plans: time the available B and B^T kernel shapes on the
-  // plan's own nodes once and keep the fastest (kernel variant overrides in
-  // the environment win)
+  // FFTW_MEASURE plans: time, on the plan's own nodes, the bin edges (B and
+  // B^T together), the B and B^T kernel shapes, the spreading work-item size
+  // and the FFT lines per workgroup once, and keep the fastest of each
+  // (overrides in the environment win)
   bool tuned = false;
   void autotune();
   void trafo(const cplx<R> *fhat_dev, cplx<R> *f_dev);

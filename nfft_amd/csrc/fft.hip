@@ -196,6 +196,61 @@ __device__ __forceinline__ void stage_inplace(cplx<R> *__restrict__ buf, int n, 
   __syncthreads();
 }
 
+// A Stockham stage of any radix (a prime factor above 13, known only at run
+// time), same math as stage_inplace: each output of a butterfly is its own
+// sum over the RAD inputs, read from LDS, with the radix-RAD DFT factor
+// exp(sign 2 pi i r k / RAD) taken from the line's twiddle table at
+// (r k mod RAD) n / RAD.  All outputs are held in registers until every
+// thread has read its inputs, then written.  O(n RAD) work per line.
+template <typename R, int SIGN, int CAP>
+__device__ __forceinline__ void stage_generic(cplx<R> *__restrict__ buf, int n, int ls, int nlines,
+                                              int Ns, int RAD, const cplx<R> *tw) {
+  constexpr int PER = CAP / 256;
+  const int q = n / RAD;
+  const int span = Ns * RAD;
+  const int twstep = n / span;
+  const int tot = nlines * n;
+  cplx<R> out[PER];
+  int dll[PER], dpos[PER];
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int o = threadIdx.x + u * 256;
+    dll[u] = -1;
+    if (o < tot) {
+      const int ll = o / n;
+      const int pos = o - ll * n;
+      // output pos = jd span + jm + k Ns of butterfly j = jd Ns + jm
+      const int jd = pos / span;
+      const int rr = pos - jd * span;
+      const int k = rr / Ns;
+      const int jm = rr - k * Ns;
+      const int j = jd * Ns + jm;
+      R sr = 0, si = 0;
+      for (int r = 0; r < RAD; r++) {
+        cplx<R> x = buf[lds_idx(ll, j + r * q, ls)];
+        // stage twiddle w^(r jm) and the DFT factor, one table entry each
+        cplx<R> w = tw[jm * r * twstep];
+        cplx<R> f = tw[((r * k) % RAD) * q];
+        if (SIGN > 0) {
+          w.y = -w.y;
+          f.y = -f.y;
+        }
+        x = cmul<R>(cmul<R>(x, w), f);
+        sr += x.x;
+        si += x.y;
+      }
+      out[u] = cplx<R>{sr, si};
+      dll[u] = ll;
+      dpos[u] = pos;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PER; u++)
+    if (dll[u] >= 0) buf[lds_idx(dll[u], dpos[u], ls)] = out[u];
+  __syncthreads();
+}
+
 // Band position of grid index k along a dimension (n, N): the f_hat
 // coordinate ks in [0, N) with k = (ks - N/2) mod n, or -1 outside the band.
 __device__ __forceinline__ int band_coord(int k, int n, int N) {
@@ -351,7 +406,7 @@ __global__ __launch_bounds__(256) void k_fft_pass(const cplx<R> *gsrc, cplx<R> *
         case 8: stage_inplace<R, 8, SIGN, CAP, false>(buf, n, ls, TL, Ns, tw); break;
         case 11: stage_inplace<R, 11, SIGN, CAP, false>(buf, n, ls, TL, Ns, tw); break;
         case 13: stage_inplace<R, 13, SIGN, CAP, false>(buf, n, ls, TL, Ns, tw); break;
-        default: break;  // excluded at plan time
+        default: stage_generic<R, SIGN, CAP>(buf, n, ls, TL, Ns, rad, tw); break;
       }
     }
     Ns *= rad;
@@ -407,7 +462,8 @@ void launch_fft_pass(cplx<R> *g, const cplx<R> *fhat_in, cplx<R> *fhat_out, cons
   FftPass<R> p = p0;
   {
     const long fit = p0.lines_per_block;
-    const long want = std::max<long>(1, p0.nlines * (long)nb / 1024);
+    long want = std::max<long>(1, p0.nlines * (long)nb / 1024);
+    want = p0.lpb_shift >= 0 ? want << p0.lpb_shift : std::max<long>(1, want >> -p0.lpb_shift);
     const long floor_lines = std::min<long>(fit, p0.stride == 1 ? 2 : 4);
     p.lines_per_block = (int)std::max(floor_lines, std::min(fit, want));
     const size_t line_bytes = sizeof(cplx<R>) * (size_t)p.lines_per_block * lds_line_stride(p.len);

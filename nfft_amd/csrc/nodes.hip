@@ -248,10 +248,10 @@ __global__ void k_sub_order(const int4 *__restrict__ in, const int *__restrict__
   out[atomicAdd(&bucket_next[bucket_of[i]], 1)] = in[i];
 }
 
-// Interpolation slots: a slot is one node, or two consecutive nodes of one
-// cell run (same cell, same work item) whose grid taps the kernel reads once.
-// Runs start where the cell changes or a work item starts; slots start at
-// the even positions of each run.
+// Interpolation slots: up to Q consecutive nodes of one cell run (same
+// cell, same work item) whose grid taps the kernel reads once.  Runs start
+// where the cell changes or a work item starts; slots start at the run
+// positions that are multiples of Q; a slot is (first node, node count).
 __global__ void k_mark_item_starts(const int4 *__restrict__ subs, long nsub, int *__restrict__ b) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nsub) b[subs[i].y] = 1;
@@ -276,18 +276,20 @@ __global__ void k_run_starts(const int *__restrict__ b, int *__restrict__ run_id
 }
 
 __global__ void k_slot_flags(const int *__restrict__ run_id, const int *__restrict__ run_start,
-                             long M, int *__restrict__ sflag) {
+                             long M, int Q, int *__restrict__ sflag) {
   long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= M) return;
-  sflag[j] = ((j - run_start[run_id[j]]) & 1) == 0 ? 1 : 0;
+  sflag[j] = ((j - run_start[run_id[j]]) % Q) == 0 ? 1 : 0;
 }
 
 __global__ void k_fill_slots(const int *__restrict__ b, const int *__restrict__ sflag,
-                             const int *__restrict__ sidx, long M, int2 *__restrict__ slots) {
+                             const int *__restrict__ sidx, long M, int Q,
+                             int2 *__restrict__ slots) {
   long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= M || !sflag[j]) return;
-  const bool pair = j + 1 < M && b[j + 1] == 0;
-  slots[sidx[j]] = make_int2((int)j, pair ? (int)(j + 1) : -1);
+  int cnt = 1;
+  while (cnt < Q && j + cnt < M && b[j + cnt] == 0) cnt++;
+  slots[sidx[j]] = make_int2((int)j, cnt);
 }
 
 __global__ void k_item_slots(const int4 *__restrict__ subs, long nsub, const int *__restrict__ sidx,
@@ -442,7 +444,10 @@ void bin_nodes(Engine<R> &e) {
     hipLaunchKernelGGL(k_run_flags, dim3(gm), dim3(256), 0, s, e.cell, gg.d, M, b);
     long_scan(b, run_id, M, sums + 100);
     hipLaunchKernelGGL(k_run_starts, dim3(gm), dim3(256), 0, s, b, run_id, M, run_start);
-    hipLaunchKernelGGL(k_slot_flags, dim3(gm), dim3(256), 0, s, run_id, run_start, M, sflag);
+    // quads only where the kernel has their code (the shared symmetric
+    // window table); otherwise pairs
+    const int Q = e.win.poly_shared && e.sym_window ? e.slot_nodes : 2;
+    hipLaunchKernelGGL(k_slot_flags, dim3(gm), dim3(256), 0, s, run_id, run_start, M, Q, sflag);
     long_scan(sflag, sidx, M, sums + 101);
     int nslots = 0;
     NFFT_HIP_CHECK(R, hipMemcpyAsync(&nslots, sums + 101, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -457,7 +462,7 @@ void bin_nodes(Engine<R> &e) {
       NFFT_HIP_CHECK(R, hipMalloc(&e.islots, sizeof(int2) * (size_t)e.nsub));
       e.islots_cap = e.nsub;
     }
-    hipLaunchKernelGGL(k_fill_slots, dim3(gm), dim3(256), 0, s, b, sflag, sidx, M, e.slots);
+    hipLaunchKernelGGL(k_fill_slots, dim3(gm), dim3(256), 0, s, b, sflag, sidx, M, Q, e.slots);
     hipLaunchKernelGGL(k_item_slots, dim3((unsigned)ceil_div(e.nsub, 256)), dim3(256), 0, s,
                        e.subs, e.nsub, sidx, M, nslots, e.islots);
     e.nslots = nslots;

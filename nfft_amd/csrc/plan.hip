@@ -13,6 +13,7 @@
 // are taken from x at precompute time when a PRE_* psi flag is set, and from
 // the current x on every call otherwise, as the reference does.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -144,7 +145,9 @@ static void fit_window_polys(int m, long double b, std::vector<R> &out) {
       out[first + w * (DEG + 1) + i] = (i % 2 ? -1 : 1) * out[first + (2 * m + 1 - w) * (DEG + 1) + i];
 }
 
-// factor n into FFT radices, largest first; returns false if a prime > 13 remains
+// factor n into FFT radices: 8, 4, 2, the small primes with unrolled
+// butterflies, then any other prime factor (a generic stage, O(n p) per
+// line); at most 20 stages
 static bool fft_radices(long n, std::vector<int> &out) {
   out.clear();
   while (n % 8 == 0 && n > 1) { out.push_back(8); n /= 8; }
@@ -152,7 +155,125 @@ static bool fft_radices(long n, std::vector<int> &out) {
   while (n % 2 == 0 && n > 1) { out.push_back(2); n /= 2; }
   for (int p : {3, 5, 7, 11, 13})
     while (n % p == 0) { out.push_back(p); n /= p; }
-  return n == 1;
+  for (long p = 17; p * p <= n; p += 2)
+    while (n % p == 0) { out.push_back((int)p); n /= p; }
+  if (n > 1) out.push_back((int)n);
+  return out.size() <= 20;
+}
+
+// Bin edges bs[t] (and with them the LDS tile edge bs + 2m + 1): the
+// geometry shared by the node binning and the B / B^T kernels, whether B^T
+// reduces its tiles through scratch, and the zone tables of that reduction.
+// Node data must be rebuilt after a change (build_nodes).
+template <typename R>
+void Engine<R>::set_bins(const int *bs) {
+  geom.d = d;
+  geom.m = m;
+  geom.M = M;
+  for (int t = 0; t < kMaxDim; t++) {
+    geom.n[t] = t < d ? (int)n[t] : 1;
+    geom.bs[t] = t < d ? bs[t] : 1;
+    geom.nb[t] = t < d ? (int)ceil_div(n[t], bs[t]) : 1;
+    geom.E[t] = t < d ? bs[t] + 2 * m + 1 : 1;
+  }
+  geom.ip2 = geom.E[2];
+  if (d == 3) {
+    // interpolation tile rows padded to 4 (mod 8) cells (an 8 x 8 plane of
+    // distinct cells then maps to 32 banks of 8 bytes 2-way instead of 4-way
+    // at 17 cells), when the tile still fits; NFFT_HIP_IPITCH=0: unpadded
+    const char *ip = std::getenv("NFFT_HIP_IPITCH");
+    if (!(ip && std::strcmp(ip, "0") == 0)) {
+      // (at most 3 cells of padding per row: measured at m = 4, 17 -> 20)
+      int p2 = geom.E[2];
+      while (p2 % 8 != 4) p2++;
+      if (p2 - geom.E[2] <= 3 && (size_t)geom.E[0] * geom.E[1] * p2 * sizeof(cplx<R>) <= 64 * 1024)
+        geom.ip2 = p2;
+    }
+  }
+  // B^T writes its LDS tiles to scratch and reduces them without atomics
+  // when the tile fits LDS and never wraps onto itself (E_t <= n_t)
+  {
+    size_t tb = sizeof(cplx<R>);
+    bool fits = true;
+    for (int t = 0; t < d; t++) {
+      tb *= (size_t)geom.E[t];
+      fits = fits && geom.E[t] <= n[t];
+    }
+    // (the tile reductions cover d <= 3; d = 4 adds its tiles with atomics)
+    tile_scratch = fits && tb <= 64 * 1024 && d <= 3;
+    bool zoned = tile_scratch;
+    long core = 1;
+    for (int t = 0; t < d; t++) {
+      zoned = zoned && geom.bs[t] >= m + 1 && n[t] % geom.bs[t] == 0;
+      core *= geom.bs[t];
+    }
+    zoned = zoned && core <= 4096;  // the zone reduction's cells per workgroup
+    if (const char *z = std::getenv("NFFT_HIP_ZONES")) zoned = zoned && std::strcmp(z, "0") != 0;
+    geom.zoned = zoned ? 1 : 0;
+    geom.zmap = geom.zdesc = nullptr;
+    if (zone_tab) NFFT_HIP_CHECK(R, hipFree(zone_tab));
+    zone_tab = nullptr;
+    if (zoned) {
+      // per dimension the tile splits into the part in the left neighbour's
+      // core (m cells), the own core (bs), the part in the right
+      // neighbour's core (m + 1); zones are written in base-3 order of
+      // their (z_0, ..., z_{d-1}) digits, each row-major
+      auto zstart = [&](int t, int z) { return z == 0 ? 0 : (z == 1 ? m : m + geom.bs[t]); };
+      auto zsize = [&](int t, int z) { return z == 0 ? m : (z == 1 ? geom.bs[t] : m + 1); };
+      int zc = 1;
+      for (int t = 0; t < d; t++) zc *= 3;
+      std::vector<int> tab;
+      std::vector<long> zbase(zc + 1, 0);
+      for (int zz = 0; zz < zc; zz++) {
+        int dg[kMaxDim] = {0, 0, 0, 0}, r = zz;
+        for (int t = d - 1; t >= 0; t--) {
+          dg[t] = r % 3;
+          r /= 3;
+        }
+        long cnt = 1;
+        for (int t = 0; t < d; t++) cnt *= zsize(t, dg[t]);
+        for (long q = 0; q < cnt; q++) {
+          long rem = q, src = 0, stride = 1;
+          for (int t = d - 1; t >= 0; t--) {
+            const int sz = zsize(t, dg[t]);
+            src += (zstart(t, dg[t]) + rem % sz) * stride;
+            rem /= sz;
+            stride *= geom.E[t];
+          }
+          tab.push_back((int)src);
+        }
+        zbase[zz + 1] = zbase[zz] + cnt;
+      }
+      const size_t nmap = tab.size();
+      // neighbour k with digits dg (0 left, 1 self, 2 right) contributes
+      // its zone 2 - dg_t per dimension, landing at core offset bs - m
+      // (right neighbour's left part goes to the top of the core) or 0
+      for (int k = 0; k < zc; k++) {
+        int dg[kMaxDim] = {1, 1, 1, 1}, r = k;
+        for (int t = d - 1; t >= 0; t--) {
+          dg[t] = r % 3;
+          r /= 3;
+        }
+        int zidx = 0;
+        int off[3] = {0, 0, 0}, size[3] = {1, 1, 1};
+        for (int t = 0; t < d; t++) {
+          const int z = 2 - dg[t];
+          zidx = zidx * 3 + z;
+          size[t] = zsize(t, z);
+          off[t] = dg[t] == 2 ? geom.bs[t] - m : 0;
+        }
+        tab.push_back((int)zbase[zidx]);
+        for (int t = 0; t < 3; t++) tab.push_back(off[t]);
+        for (int t = 0; t < 3; t++) tab.push_back(size[t]);
+        tab.push_back(0);
+      }
+      NFFT_HIP_CHECK(R, hipMalloc(&zone_tab, sizeof(int) * tab.size()));
+      NFFT_HIP_CHECK(R, hipMemcpy(zone_tab, tab.data(), sizeof(int) * tab.size(),
+                                  hipMemcpyHostToDevice));
+      geom.zmap = zone_tab;
+      geom.zdesc = zone_tab + nmap;
+    }
+  }
 }
 
 template <typename R>
@@ -245,29 +366,17 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     if (bs[tmax] == 1) break;
     bs[tmax] = (bs[tmax] + 1) / 2;
   }
-  geom.d = d;
-  geom.m = m;
-  geom.M = M;
-  for (int t = 0; t < kMaxDim; t++) {
-    geom.n[t] = t < d ? (int)n[t] : 1;
-    geom.bs[t] = t < d ? bs[t] : 1;
-    geom.nb[t] = t < d ? (int)ceil_div(n[t], bs[t]) : 1;
-    geom.E[t] = t < d ? bs[t] + 2 * m + 1 : 1;
-  }
-  geom.ip2 = geom.E[2];
-  if (d == 3) {
-    // interpolation tile rows padded to 4 (mod 8) cells (an 8 x 8 plane of
-    // distinct cells then maps to 32 banks of 8 bytes 2-way instead of 4-way
-    // at 17 cells), when the tile still fits; NFFT_HIP_IPITCH=0: unpadded
-    const char *ip = std::getenv("NFFT_HIP_IPITCH");
-    if (!(ip && std::strcmp(ip, "0") == 0)) {
-      // (at most 3 cells of padding per row: measured at m = 4, 17 -> 20)
-      int p2 = geom.E[2];
-      while (p2 % 8 != 4) p2++;
-      if (p2 - geom.E[2] <= 3 && (size_t)geom.E[0] * geom.E[1] * p2 * sizeof(cplx<R>) <= 64 * 1024)
-        geom.ip2 = p2;
+  // NFFT_HIP_BINS=e0,e1,...: bin edges instead of the default
+  if (const char *v = std::getenv("NFFT_HIP_BINS")) {
+    int k = 0;
+    for (const char *q = v; *q && k < d; k++) {
+      const int e = std::atoi(q);
+      if (e >= 1) bs[k] = (int)std::min<long>(n[k], e);
+      while (*q && *q != ',') q++;
+      if (*q == ',') q++;
     }
   }
+  set_bins(bs);
   // nodes per work item; larger 3-D spreading items write fewer tiles for
   // the reduction (measured on the 3-D kooshball: 2048 -> 4096 halves the
   // tile reduction at equal spreading time, larger items end the launch late)
@@ -285,88 +394,6 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   if (const char *v = std::getenv("NFFT_HIP_SPREAD_ITEM"))
     spread_max_per_sub = std::max(64, std::atoi(v));
   if (const char *v = std::getenv("NFFT_HIP_SPREAD_CAP")) spread_max_items = std::max(0, std::atoi(v));
-  // B^T writes its LDS tiles to scratch and reduces them without atomics
-  // when the tile fits LDS and never wraps onto itself (E_t <= n_t)
-  {
-    size_t tb = sizeof(cplx<R>);
-    bool fits = true;
-    for (int t = 0; t < d; t++) {
-      tb *= (size_t)geom.E[t];
-      fits = fits && geom.E[t] <= n[t];
-    }
-    // (the tile reductions cover d <= 3; d = 4 adds its tiles with atomics)
-    tile_scratch = fits && tb <= 64 * 1024 && d <= 3;
-    bool zoned = tile_scratch;
-    long core = 1;
-    for (int t = 0; t < d; t++) {
-      zoned = zoned && geom.bs[t] >= m + 1 && n[t] % geom.bs[t] == 0;
-      core *= geom.bs[t];
-    }
-    zoned = zoned && core <= 4096;  // the zone reduction's cells per workgroup
-    if (const char *z = std::getenv("NFFT_HIP_ZONES")) zoned = zoned && std::strcmp(z, "0") != 0;
-    geom.zoned = zoned ? 1 : 0;
-    geom.zmap = geom.zdesc = nullptr;
-    if (zoned) {
-      // per dimension the tile splits into the part in the left neighbour's
-      // core (m cells), the own core (bs), the part in the right
-      // neighbour's core (m + 1); zones are written in base-3 order of
-      // their (z_0, ..., z_{d-1}) digits, each row-major
-      auto zstart = [&](int t, int z) { return z == 0 ? 0 : (z == 1 ? m : m + geom.bs[t]); };
-      auto zsize = [&](int t, int z) { return z == 0 ? m : (z == 1 ? geom.bs[t] : m + 1); };
-      int zc = 1;
-      for (int t = 0; t < d; t++) zc *= 3;
-      std::vector<int> tab;
-      std::vector<long> zbase(zc + 1, 0);
-      for (int zz = 0; zz < zc; zz++) {
-        int dg[kMaxDim] = {0, 0, 0, 0}, r = zz;
-        for (int t = d - 1; t >= 0; t--) {
-          dg[t] = r % 3;
-          r /= 3;
-        }
-        long cnt = 1;
-        for (int t = 0; t < d; t++) cnt *= zsize(t, dg[t]);
-        for (long q = 0; q < cnt; q++) {
-          long rem = q, src = 0, stride = 1;
-          for (int t = d - 1; t >= 0; t--) {
-            const int sz = zsize(t, dg[t]);
-            src += (zstart(t, dg[t]) + rem % sz) * stride;
-            rem /= sz;
-            stride *= geom.E[t];
-          }
-          tab.push_back((int)src);
-        }
-        zbase[zz + 1] = zbase[zz] + cnt;
-      }
-      const size_t nmap = tab.size();
-      // neighbour k with digits dg (0 left, 1 self, 2 right) contributes
-      // its zone 2 - dg_t per dimension, landing at core offset bs - m
-      // (right neighbour's left part goes to the top of the core) or 0
-      for (int k = 0; k < zc; k++) {
-        int dg[kMaxDim] = {1, 1, 1, 1}, r = k;
-        for (int t = d - 1; t >= 0; t--) {
-          dg[t] = r % 3;
-          r /= 3;
-        }
-        int zidx = 0;
-        int off[3] = {0, 0, 0}, size[3] = {1, 1, 1};
-        for (int t = 0; t < d; t++) {
-          const int z = 2 - dg[t];
-          zidx = zidx * 3 + z;
-          size[t] = zsize(t, z);
-          off[t] = dg[t] == 2 ? geom.bs[t] - m : 0;
-        }
-        tab.push_back((int)zbase[zidx]);
-        for (int t = 0; t < 3; t++) tab.push_back(off[t]);
-        for (int t = 0; t < 3; t++) tab.push_back(size[t]);
-        tab.push_back(0);
-      }
-      NFFT_HIP_CHECK(R, hipMalloc(&zone_tab, sizeof(int) * tab.size()));
-      NFFT_HIP_CHECK(R, hipMemcpy(zone_tab, tab.data(), sizeof(int) * tab.size(),
-                                  hipMemcpyHostToDevice));
-      geom.zmap = zone_tab;
-      geom.zdesc = zone_tab + nmap;
-    }
-  }
   auto variant = [](const char *name, int dflt) {
     const char *v = std::getenv(name);
     if (!v) return dflt;
@@ -382,6 +409,7 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   interp_variant = variant("NFFT_HIP_INTERP", 0);
   if (const char *v = std::getenv("NFFT_HIP_SYMWIN")) sym_window = std::atoi(v) != 0;
   if (const char *v = std::getenv("NFFT_HIP_PAIRS")) pair_interp = std::atoi(v) != 0;
+  if (const char *v = std::getenv("NFFT_HIP_SLOT")) slot_nodes = std::atoi(v) >= 4 ? 4 : 2;
   // float: plain FP32 read-modify-write by wavefronts that own disjoint
   // cells, same-cell runs summed in registers (3-D: two wavefronts per tile
   // split by row parity; 2-D: one wavefront per tile; 1-D: private tiles);
@@ -433,8 +461,8 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
       if (!best || std::labs(a - b) < std::labs(best - n[t] / best)) best = a;
     }
     if (!best)
-      family<R>::fatal("nfft: FFT length n[%d] = %ld is neither at most %d with prime factors "
-                       "up to 13 nor a product of two such lengths", t, n[t], cap);
+      family<R>::fatal("nfft: FFT length n[%d] = %ld is above %d and not a product of two "
+                       "lengths up to %d", t, n[t], cap, cap);
     split[t] = best;
     axes.push_back(FftAxis<R>{});  // placeholder, the halves carry the tables
     axes.back().len = (int)n[t];
@@ -551,6 +579,12 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     cur = 0;
     for (int t = 0; t < d; t++)
       halves(make_pass(t, kFromGrid, t == d - 1 ? kToCoeffs : kToGridBand), t, cur, bwd_passes);
+    // NFFT_HIP_FFT_LINES=k: lines per workgroup times 2^k in every pass
+    if (const char *v = std::getenv("NFFT_HIP_FFT_LINES")) {
+      const int sh = std::max(-4, std::min(4, std::atoi(v)));
+      for (auto &ps : fwd_passes) ps.lpb_shift = sh;
+      for (auto &ps : bwd_passes) ps.lpb_shift = sh;
+    }
   }
 }
 
@@ -809,6 +843,62 @@ void Engine<R>::autotune() {
     return ms;
   };
   // inputs: the device image of f (its values do not change the timing)
+  // bin edges: the default, and the largest edge halved; in 1-D and 2-D also
+  // every edge doubled and every edge halved; all within the LDS budget of
+  // the default, nodes rebuilt for each, B and B^T timed together
+  if (!std::getenv("NFFT_HIP_BINS") && d <= 3) {
+    auto edges = [&]() {
+      std::array<int, kMaxDim> e{};
+      for (int t = 0; t < kMaxDim; t++) e[t] = geom.bs[t];
+      return e;
+    };
+    const std::array<int, kMaxDim> base = edges();
+    auto tile_of = [&](const std::array<int, kMaxDim> &e) {
+      size_t s = sizeof(cplx<R>);
+      for (int t = 0; t < d; t++) s *= (size_t)(e[t] + 2 * m + 1);
+      return s;
+    };
+    std::vector<std::array<int, kMaxDim>> cand;
+    {
+      std::array<int, kMaxDim> h = base;
+      int tmax = 0;
+      for (int t = 1; t < d; t++)
+        if (h[t] > h[tmax]) tmax = t;
+      h[tmax] = std::max(1, h[tmax] / 2);
+      cand.push_back(h);
+      if (d <= 2) {
+        std::array<int, kMaxDim> up = base, dn = base;
+        for (int t = 0; t < d; t++) {
+          up[t] = (int)std::min<long>(n[t], 2L * base[t]);
+          dn[t] = std::max(1, base[t] / 2);
+        }
+        cand.push_back(up);
+        cand.push_back(dn);
+      }
+    }
+    auto time_b = [&]() {
+      return time_of([&]() {
+        spread_to_grid(f);
+        interp_from_grid(f);
+      });
+    };
+    float best = time_b();
+    std::array<int, kMaxDim> pick = base;
+    for (const auto &c : cand) {
+      if (c == base || tile_of(c) > std::max<size_t>(tile_of(base), 48 * 1024)) continue;
+      set_bins(c.data());
+      build_nodes();
+      const float ms = time_b();
+      if (ms < best) {
+        best = ms;
+        pick = c;
+      }
+    }
+    if (pick != edges()) {
+      set_bins(pick.data());
+      build_nodes();
+    }
+  }
   if (!std::getenv("NFFT_HIP_SPREAD")) {
     // float candidates accumulate in plain FP32 (the fixed-point LDS
     // variants are left out); double ones use ds_add_f64
@@ -863,6 +953,31 @@ void Engine<R>::autotune() {
       }
     }
     interp_variant = pick;
+  }
+  // FFT lines per workgroup: half, the default and twice the default choice
+  // of the launch, one setting for all passes of both directions
+  if (!std::getenv("NFFT_HIP_FFT_LINES")) {
+    auto set_shift = [&](int sh) {
+      for (auto &ps : fwd_passes) ps.lpb_shift = sh;
+      for (auto &ps : bwd_passes) ps.lpb_shift = sh;
+    };
+    auto time_fft = [&]() {
+      return time_of([&]() {
+        for (auto &ps : fwd_passes) launch_fft_pass<R>(g, fhat, nullptr, ps, -1, stream, 1, g2);
+        for (auto &ps : bwd_passes) launch_fft_pass<R>(g, nullptr, fhat, ps, +1, stream, 1, g2);
+      });
+    };
+    float best = time_fft();
+    int pick = 0;
+    for (int sh : {-1, 1}) {
+      set_shift(sh);
+      const float ms = time_fft();
+      if (ms < best) {
+        best = ms;
+        pick = sh;
+      }
+    }
+    set_shift(pick);
   }
   hipEventDestroy(a);
   hipEventDestroy(b);

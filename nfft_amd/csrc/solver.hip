@@ -20,6 +20,7 @@
 //   v = A W_hat p; dot_v = v^H W v; alpha = dot_z / dot_v
 //   f_hat += alpha W_hat p; r -= alpha v; dot_r = r^H W r
 //   z = A^H W r; beta = dot_z_new / dot_z_old; p = beta p + z
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <unordered_map>
@@ -140,6 +141,14 @@ template <typename R> struct DevCgnr {
   double *part = nullptr, *sc = nullptr;
   double *sc_host = nullptr;  // pinned
   int sync_level = 2;         // 0 scalars, 1 + f_hat_iter, 2 + all vectors
+  // The kernels of one step (everything but the copies back to the host)
+  // as a hipGraph, captured on the first step and replayed with one launch
+  // afterwards (NFFT_HIP_CGNR_GRAPH=0 at solver creation: eager launches).
+  // Re-captured when the plan's stream changes; not used while stage timing
+  // is on (its events are created on demand).
+  bool use_graph = true;
+  hipGraphExec_t step_exec = nullptr;
+  hipStream_t step_stream = nullptr;
 
   void alloc(Engine<R> *eng, long M_, long N_, bool weighted, bool damped) {
     e = eng;
@@ -164,6 +173,7 @@ template <typename R> struct DevCgnr {
     NFFT_HIP_CHECK(R, hipMemset(sc, 0, S_COUNT * sizeof(double)));
   }
   void release() {
+    if (step_exec) hipGraphExecDestroy(step_exec);
     for (void *q : {(void *)y, (void *)r, (void *)v, (void *)tmpM, (void *)fh, (void *)z,
                     (void *)p, (void *)tmpN, (void *)w, (void *)w_hat, (void *)part, (void *)sc})
       if (q) hipFree(q);
@@ -283,6 +293,7 @@ struct SolverApi {
     if ((flags & CGNR) && e && !e->direct_only) {
       DevCgnr<R> *dv = new DevCgnr<R>();
       dv->alloc(e, M, N, flags & PRECOMPUTE_WEIGHT, flags & PRECOMPUTE_DAMP);
+      if (const char *gv = std::getenv("NFFT_HIP_CGNR_GRAPH")) dv->use_graph = std::strcmp(gv, "0") != 0;
       auto &rg = registry<R>();
       std::lock_guard<std::mutex> g(rg.mu);
       rg.map[(void *)ths] = dv;
@@ -353,6 +364,30 @@ struct SolverApi {
   static void dev_step(SPLAN *ths, DevCgnr<R> *dv) {
     Engine<R> *e = dv->e;
     hipStream_t s = e->stream;
+    if (dv->use_graph && !e->timing && !e->deferred) {
+      if (!dv->step_exec || dv->step_stream != s) {
+        if (dv->step_exec) NFFT_HIP_CHECK(R, hipGraphExecDestroy(dv->step_exec));
+        dv->step_exec = nullptr;
+        hipGraph_t graph;
+        NFFT_HIP_CHECK(R, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        dev_step_kernels(dv);
+        NFFT_HIP_CHECK(R, hipStreamEndCapture(s, &graph));
+        NFFT_HIP_CHECK(R, hipGraphInstantiate(&dv->step_exec, graph, nullptr, nullptr, 0));
+        NFFT_HIP_CHECK(R, hipGraphDestroy(graph));
+        dv->step_stream = s;
+      }
+      NFFT_HIP_CHECK(R, hipGraphLaunch(dv->step_exec, s));
+    } else {
+      dev_step_kernels(dv);
+    }
+    sync_out(ths, dv);
+  }
+
+  // the kernels of one CGNR step, enqueued on the plan's stream: no
+  // allocation, host copy or memset, so the sequence can be captured
+  static void dev_step_kernels(DevCgnr<R> *dv) {
+    Engine<R> *e = dv->e;
+    hipStream_t s = e->stream;
     const long M = dv->M, N = dv->N;
     const unsigned bM = blocks_for(M), bN = blocks_for(N);
     // v = A W_hat p
@@ -376,7 +411,6 @@ struct SolverApi {
     // p = beta p + z
     if (N) hipLaunchKernelGGL(k_axpy_beta<R>, dim3(bN), dim3(256), 0, s, dv->p, dv->sc, dv->z, N);
     NFFT_HIP_CHECK(R, hipGetLastError());
-    sync_out(ths, dv);
   }
 
   // ---------------------------------------------------------------- host path

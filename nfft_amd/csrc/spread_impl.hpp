@@ -280,9 +280,103 @@ __device__ __forceinline__ void load_region_3d(const cplx<R> *__restrict__ g, cp
   }
 }
 
+// Interpolation of one slot of up to four nodes of one cell (3-D, FP32, the
+// shared symmetric window table): every grid tap of the slot's window is
+// read once from LDS for all its nodes.  The dimension-1 and -2 windows of
+// the nodes stay in registers; the dimension-0 weights are evaluated row
+// pair by row pair (rows i and W-1-i from one polynomial's even and odd
+// parts), which keeps the kernel at three wavefronts per SIMD.  Nodes past
+// the slot's count repeat its last node and are not written; a lane with
+// live = false writes nothing.
+template <int W>
+__device__ __forceinline__ void interp_quad_3d(const float2 *tile, float2 *__restrict__ fout,
+                                               const int *__restrict__ perm,
+                                               const int *__restrict__ cell,
+                                               const float *__restrict__ frac, const BinGeom &gg,
+                                               const WinParams<float> &wp, const int (&lo)[3],
+                                               int E1, int E2, int2 pr, bool live) {
+  constexpr int KE = kPolyDeg - (kPolyDeg % 2);      // highest even power
+  constexpr int KO = kPolyDeg - 1 + (kPolyDeg % 2);  // highest odd power
+  const long M = gg.M;
+  long pn[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) pn[q] = (long)pr.x + min(q, pr.y - 1);
+  // the symmetric tap pair (w, W-1-w) at tt = frac - 1/2
+  auto tap_pair = [&](int w, float tt, float &lo_v, float &hi_v) {
+    const float *pc = wp.poly + w * kPolyCoef;
+    const float t2 = tt * tt;
+    float e = pc[KE];
+#pragma unroll
+    for (int k = KE - 2; k >= 0; k -= 2) e = fmaf(e, t2, pc[k]);
+    float o = pc[KO];
+#pragma unroll
+    for (int k = KO - 2; k >= 1; k -= 2) o = fmaf(o, t2, pc[k]);
+    lo_v = fmaf(tt, o, e);
+    hi_v = fmaf(-tt, o, e);
+  };
+  float p1[4][W], p2[4][W], t0[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    t0[q] = frac[pn[q]] - 0.5f;
+    const float t1 = frac[M + pn[q]] - 0.5f;
+    const float t2 = frac[2 * M + pn[q]] - 0.5f;
+#pragma unroll
+    for (int w = 0; w < W / 2; w++) {
+      tap_pair(w, t1, p1[q][w], p1[q][W - 1 - w]);
+      tap_pair(w, t2, p2[q][w], p2[q][W - 1 - w]);
+    }
+  }
+  const float2 *base = tile + ((cell[pr.x] - lo[0]) * E1 + (cell[M + pr.x] - lo[1])) * E2 +
+                       (cell[2 * M + pr.x] - lo[2]);
+  float ar[4] = {0.f, 0.f, 0.f, 0.f}, ai[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int ip = 0; ip < W / 2; ip++) {
+    float w0[2][4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) tap_pair(ip, t0[q], w0[0][q], w0[1][q]);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int i0 = h == 0 ? ip : W - 1 - ip;
+      float qr[4] = {0.f, 0.f, 0.f, 0.f}, qi[4] = {0.f, 0.f, 0.f, 0.f};
+      for (int i1 = 0; i1 < W; i1++) {
+        const float2 *row = base + (i0 * E1 + i1) * E2;
+        // the row's taps are all requested before the first is used
+        float2 v[W];
+#pragma unroll
+        for (int w = 0; w < W; w++) v[w] = row[w];
+        float sr[4] = {0.f, 0.f, 0.f, 0.f}, si[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int w = 0; w < W; w++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            sr[q] = fmaf(p2[q][w], v[w].x, sr[q]);
+            si[q] = fmaf(p2[q][w], v[w].y, si[q]);
+          }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          qr[q] = fmaf(p1[q][i1], sr[q], qr[q]);
+          qi[q] = fmaf(p1[q][i1], si[q], qi[q]);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        ar[q] = fmaf(w0[h][q], qr[q], ar[q]);
+        ai[q] = fmaf(w0[h][q], qi[q], ai[q]);
+      }
+    }
+  }
+  if (live) {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (q < pr.y) fout[perm[pr.x + q]] = float2{ar[q], ai[q]};
+  }
+}
+
 // PAIR (D = 2, 3): threads take the work item's slots (gg.islots / gg.slots)
 // instead of single nodes; a slot of two nodes of one cell reads each grid
 // tap once from LDS for both, halving the LDS bytes per node of dense runs.
+// With SYM in 3-D a slot may hold up to four nodes: a wavefront whose slots
+// hold at most two runs the pair code, otherwise the quad code
+// (interp_quad_3d), which reads each tap once for four nodes.
 template <typename R, int D, int W, bool SYM = false, bool PAIR = false>
 __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__ g,
                                                      cplx<R> *__restrict__ fout,
@@ -314,11 +408,26 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
   __syncthreads();
   if constexpr (PAIR && (D == 2 || D == 3)) {
     const int2 is = gg.islots[blockIdx.x];
-    for (int k = threadIdx.x; k < is.y; k += blockDim.x) {
-      const int2 pr = gg.slots[is.x + k];
+    for (int k0 = 0; k0 < is.y; k0 += blockDim.x) {
+      const int k = k0 + threadIdx.x;
+      // (first node, count) of this lane's slot; lanes past the item's
+      // slots take its first slot and write nothing
+      const int2 pr = gg.slots[is.x + (k < is.y ? k : 0)];
+      const bool live = k < is.y;
+      if constexpr (D == 3 && SYM) {
+        // the wavefront's largest slot picks the code path
+        int wmax = live ? pr.y : 0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
+        if (wmax > 2) {
+          interp_quad_3d<W>(tile, fout, perm, cell, frac, gg, wp, lo, E1, E2, pr, live);
+          continue;
+        }
+      }
+      if (!live) continue;
       const long pa = pr.x;
-      const bool two = pr.y >= 0;
-      const long pb = two ? pr.y : pr.x;
+      const bool two = pr.y >= 2;
+      const long pb = two ? pa + 1 : pa;
       int c[D], cb[D];
       R pa_[D][W], pb_[D][W];
       if constexpr (SYM) {

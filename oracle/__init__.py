@@ -58,6 +58,9 @@ def lib():
             getattr(L, name).argtypes = [ctypes.c_int, lp, lp, ctypes.c_long, dp, dp]
         for name in ("oracle_B", "oracle_BT", "oracle_nfft_trafo", "oracle_nfft_adjoint"):
             getattr(L, name).argtypes = [ctypes.c_int, lp, lp, ctypes.c_long, ctypes.c_long, dp, dp, dp]
+        L.oracle_prepsi_3d.argtypes = [lp, lp, ctypes.c_long, ctypes.c_long, dp, dp, lp]
+        for name in ("oracle_B_3d_prepsi", "oracle_BT_3d_prepsi"):
+            getattr(L, name).argtypes = [lp, ctypes.c_long, ctypes.c_long, dp, lp, dp, dp]
         _LIB = L
     return _LIB
 
@@ -173,6 +176,36 @@ def stage_BT(N, n, m, x, f):
     g = np.empty(int(np.prod(n)), dtype=np.complex128)
     lib().oracle_BT(d, _lp(N), _lp(n), int(m), x.shape[0], _dp(x), _dp(f), _dp(g))
     return g
+
+
+class Prepsi3d:
+    """3-D B / B^T in the shape of the reference's PRE_PSI path (window values
+    precomputed per node, tensor loops, blockwise adjoint), for timing the
+    CPU baseline; equal to stage_B / stage_BT to rounding."""
+
+    def __init__(self, N, n, m, x):
+        self.N, self.n, self.m = list(N), list(n), int(m)
+        self.x = _x(x, 3)
+        M = self.x.shape[0]
+        w = 2 * self.m + 2
+        self.psi = np.empty(M * 3 * w)
+        self.u = np.empty(M * 3, dtype=np.int64)
+        lib().oracle_prepsi_3d(_lp(N), _lp(n), self.m, M, _dp(self.x), _dp(self.psi),
+                               self.u.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+
+    def B(self, g):
+        g = _c128(g).ravel()
+        f = np.empty(self.x.shape[0], dtype=np.complex128)
+        lib().oracle_B_3d_prepsi(_lp(self.n), self.m, self.x.shape[0], _dp(self.psi),
+                                 self.u.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), _dp(g), _dp(f))
+        return f
+
+    def BT(self, f):
+        f = _c128(f).ravel()
+        g = np.empty(int(np.prod(self.n)), dtype=np.complex128)
+        lib().oracle_BT_3d_prepsi(_lp(self.n), self.m, self.x.shape[0], _dp(self.psi),
+                                  self.u.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), _dp(f), _dp(g))
+        return g
 
 
 def nfft_trafo(N, n, m, x, f_hat):

@@ -24,6 +24,7 @@
  *                            adjointness <A fh, f> = <fh, A^H f>
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -548,6 +549,112 @@ static int use_direct(int d, const long *N, const long *n, long m)
     if (N[t] <= m || n[t] <= 2 * m + 2)
       return 1;
   return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 3-D B / B^T shaped as the reference's PRE_PSI path, for the CPU baseline  */
+/* timing only (results equal oracle_B / oracle_BT to rounding):             */
+/*   precompute: psi[j][t][l] = phi(n_t, x_t - (u_t + l)/n_t) and the start  */
+/*     u_t = floor(n_t x_t) - m per node (kernel/nfft/nfft.c:5819-5844)      */
+/*   trafo:  per node, tensor loops l0, l1 (weights multiplied outside) and  */
+/*     a contiguous run over l2 (kernel/nfft/nfft.c:4020-4265, 4687-4914),   */
+/*     OpenMP over nodes                                                     */
+/*   adjoint: blockwise over the first grid dimension, as                    */
+/*     NFFT_OMP_BLOCKWISE_ADJOINT (kernel/nfft/nfft.c:5126-5384, 4289-4685): */
+/*     thread k owns grid planes [k n0 / T, (k+1) n0 / T) and adds the taps  */
+/*     of every node that fall in its planes, so no atomics are needed      */
+/* ------------------------------------------------------------------------ */
+void oracle_prepsi_3d(const long *N, const long *n, long m, long M, const double *x,
+                      double *psi, long *u)
+{
+  const long w = 2 * m + 2;
+  double b[3];
+  for (int t = 0; t < 3; t++)
+    b[t] = oracle_kb_b((double)n[t] / (double)N[t]);
+#pragma omp parallel for num_threads(oracle_threads) schedule(static)
+  for (long j = 0; j < M; j++)
+    for (int t = 0; t < 3; t++)
+    {
+      const long c = (long)floor(x[3 * j + t] * (double)n[t]);
+      u[3 * j + t] = c - m;
+      for (long l = 0; l < w; l++)
+        psi[(j * 3 + t) * w + l] =
+            oracle_phi(n[t], x[3 * j + t] - (double)(c - m + l) / (double)n[t], m, b[t]);
+    }
+}
+
+static inline long wrapl(long v, long n) { return ((v % n) + n) % n; }
+
+void oracle_B_3d_prepsi(const long *n, long m, long M, const double *psi, const long *u,
+                        const double *g, double *f)
+{
+  const long w = 2 * m + 2;
+#pragma omp parallel for num_threads(oracle_threads) schedule(static, 1024)
+  for (long j = 0; j < M; j++)
+  {
+    const double *p0 = psi + (j * 3 + 0) * w, *p1 = psi + (j * 3 + 1) * w,
+                 *p2 = psi + (j * 3 + 2) * w;
+    double re = 0.0, im = 0.0;
+    for (long l0 = 0; l0 < w; l0++)
+    {
+      const long i0 = wrapl(u[3 * j] + l0, n[0]);
+      for (long l1 = 0; l1 < w; l1++)
+      {
+        const double w01 = p0[l0] * p1[l1];
+        const long row = (i0 * n[1] + wrapl(u[3 * j + 1] + l1, n[1])) * n[2];
+        const long c0 = wrapl(u[3 * j + 2], n[2]);
+        double sr = 0.0, si = 0.0;
+        for (long l2 = 0; l2 < w; l2++)
+        {
+          long c = c0 + l2;
+          if (c >= n[2]) c -= n[2];
+          sr += p2[l2] * g[2 * (row + c)];
+          si += p2[l2] * g[2 * (row + c) + 1];
+        }
+        re += w01 * sr;
+        im += w01 * si;
+      }
+    }
+    f[2 * j] = re;
+    f[2 * j + 1] = im;
+  }
+}
+
+void oracle_BT_3d_prepsi(const long *n, long m, long M, const double *psi, const long *u,
+                         const double *f, double *g)
+{
+  const long w = 2 * m + 2;
+  memset(g, 0, sizeof(double) * 2 * (size_t)(n[0] * n[1] * n[2]));
+#pragma omp parallel num_threads(oracle_threads)
+  {
+    const int T = omp_get_num_threads(), k = omp_get_thread_num();
+    const long p_lo = k * n[0] / T, p_hi = (k + 1) * n[0] / T;
+    for (long j = 0; j < M; j++)
+    {
+      const double *p0 = psi + (j * 3 + 0) * w, *p1 = psi + (j * 3 + 1) * w,
+                   *p2 = psi + (j * 3 + 2) * w;
+      for (long l0 = 0; l0 < w; l0++)
+      {
+        const long i0 = wrapl(u[3 * j] + l0, n[0]);
+        if (i0 < p_lo || i0 >= p_hi)
+          continue;
+        const double a0r = p0[l0] * f[2 * j], a0i = p0[l0] * f[2 * j + 1];
+        for (long l1 = 0; l1 < w; l1++)
+        {
+          const double ar = a0r * p1[l1], ai = a0i * p1[l1];
+          const long row = (i0 * n[1] + wrapl(u[3 * j + 1] + l1, n[1])) * n[2];
+          long c = wrapl(u[3 * j + 2], n[2]);
+          for (long l2 = 0; l2 < w; l2++)
+          {
+            g[2 * (row + c)] += p2[l2] * ar;
+            g[2 * (row + c) + 1] += p2[l2] * ai;
+            if (++c == n[2])
+              c = 0;
+          }
+        }
+      }
+    }
+  }
 }
 
 void oracle_nfft_trafo(int d, const long *N, const long *n, long m, long M,

@@ -1,4 +1,5 @@
-"""FFT axes longer than one workgroup's LDS lines (GPU, through the C ABI).
+"""FFT axes longer than one workgroup's LDS lines, and lengths with prime
+factors above 13 (GPU, through the C ABI).
 
 An oversampled axis of length n above 4096 (float) / 2048 (double) runs as
 two four-step halves n = n1 n2 (fft.hip); FFTW, which the reference plans
@@ -10,7 +11,9 @@ FP64 oracle as the single-pass axes are:
     examples/nfft/nfft_benchomp.c:489) in both families on a 2000-node
     sample, FP32 against the FP64 plan over all nodes;
   * batched calls on such a plan (one right-hand side at a time) equal the
-    single calls.
+    single calls;
+  * lengths with a prime factor above 13 (a generic-radix stage), a prime
+    length, and a four-step split with such a factor, against the oracle.
 """
 import numpy as np
 import pytest
@@ -132,3 +135,26 @@ def test_long_axis_batch_equals_single_calls():
                          / torch.linalg.norm(oneh)) < 1e-6
     finally:
         p.finalize()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,N,n,M", [("double", [16], [34], 300),
+                                             ("double", [500], [1031], 2000),
+                                             ("double", [16, 20], [38, 46], 1000),
+                                             ("single", [1000], [2006], 3000),
+                                             ("single", [4000], [8738], 3000)],
+                         ids=["2x17", "prime1031", "2d_19_23", "fp32_2x17x59", "fp32_split_2x17x257"])
+def test_prime_factors_vs_oracle(precision, N, n, M):
+    d = len(N)
+    x = W.uniform(M, d, seed=61)
+    cd = np.complex64 if precision == "single" else np.complex128
+    NN = int(np.prod(N))
+    fh = W.complex_normal(NN, seed=62).astype(cd).astype(np.complex128)
+    f = W.complex_normal(M, seed=63).astype(cd).astype(np.complex128)
+    m = 4 if precision == "single" else 6
+    t, a, xr = run(N, n, m, x, fh, f, precision)
+    et = rel(t, O.nfft_trafo(N, n, m, xr, fh))
+    ea = rel(a, O.nfft_adjoint(N, n, m, xr, f))
+    tol = 1e-5 if precision == "single" else 1e-12
+    print("%s N=%s n=%s: trafo %.2e adjoint %.2e" % (precision, N, n, et, ea))
+    assert et < tol and ea < tol

@@ -299,14 +299,17 @@ def test_four_dimensional(precision, m, N, n):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("pairs", ["0", "1"])
+@pytest.mark.parametrize("pairs,slot", [("0", "2"), ("1", "2"), ("1", "4")])
 @pytest.mark.parametrize("pitch", ["0", "1"])
-def test_interp_pairs_on_and_off(pairs, pitch, monkeypatch):
-    """3-D FP32 interpolation with and without same-cell node pairs (dense
-    kooshball centre: most nodes share their cell) against the oracle, with
-    the interpolation tile's rows at their natural length and padded
+def test_interp_pairs_on_and_off(pairs, slot, pitch, monkeypatch):
+    """3-D FP32 interpolation without same-cell slots, with slots of up to two
+    and of up to four nodes (dense kooshball centre: most nodes share their
+    cell; the outer shell mixes cells of one to three nodes, so both the
+    pair and the quad code run, and quad slots of three) against the oracle,
+    with the interpolation tile's rows at their natural length and padded
     (NFFT_HIP_IPITCH)."""
     monkeypatch.setenv("NFFT_HIP_PAIRS", pairs)
+    monkeypatch.setenv("NFFT_HIP_SLOT", slot)
     monkeypatch.setenv("NFFT_HIP_IPITCH", pitch)
     N = [16, 16, 16]
     x = W.kooshball(96, 64, seed=5)

@@ -148,3 +148,45 @@ def test_device_cgnr_sync_levels():
     finally:
         s.finalize()
         p.finalize()
+
+
+@pytest.mark.gpu
+def test_cgnr_step_graph_matches_eager(monkeypatch):
+    """The device CGNR captures its step as a hipGraph on the first step and
+    replays it afterwards; every replay (with the host copies and waits of
+    sync_out between them) must equal the eager launches bit for bit.  Both
+    solvers run on one plan: its node order (set by the binning's atomics)
+    fixes the FP32 summation order of B^T, so separate plans may differ in
+    the last bits."""
+    N = [64, 64]
+    x = W.golden_radial(60, 128)
+    M = x.shape[0]
+    y = W.complex_normal(M, seed=81)
+    w = np.random.default_rng(82).uniform(0.5, 2.0, M)
+    p = nf.Plan(N, M, m=4, n=[128, 128], flags=FLAGS, precision="single")
+    p.x[:] = x
+    p.precompute()
+    runs = []
+    try:
+        for graph in ("0", "1", "0"):
+            monkeypatch.setenv("NFFT_HIP_CGNR_GRAPH", graph)
+            s = nf.Solver(p, nf.CGNR | nf.PRECOMPUTE_WEIGHT)
+            try:
+                s.y[:] = y
+                s.w[:] = w
+                s.f_hat_iter[:] = 0
+                s.before_loop()
+                steps = []
+                for _ in range(6):
+                    s.loop_one_step()
+                    steps.append((s.f_hat_iter.copy(), s.r_iter.copy(), float(s.s.dot_z_hat_iter),
+                                  float(s.s.alpha_iter), float(s.s.beta_iter)))
+                runs.append(steps)
+            finally:
+                s.finalize()
+    finally:
+        p.finalize()
+    for name, other in (("graph", runs[1]), ("eager again", runs[2])):
+        for it, (a, b) in enumerate(zip(runs[0], other)):
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (name, it)
+            assert a[2:] == b[2:], (name, it, a[2:], b[2:])

@@ -219,3 +219,27 @@ def test_window_outer_taps_below_golden_resolution():
     pinned instead by test_phi_sin_branch_is_analytic_continuation."""
     subprocess.run(["make", "-s", "-C", O._HERE, "liboracle_plant1.so"], check=True)
     assert _golden_pass_rate("liboracle_plant1.so") == 1.0
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+def test_prepsi_3d_equals_oracle_b(threads):
+    """The reference-shaped 3-D PRE_PSI B / B^T (the CPU baseline's timing
+    path) equal the oracle's own B / B^T, edge nodes and wrap included."""
+    N, n, m = [8, 10, 12], [16, 20, 24], 3
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-0.5, 0.5, size=(400, 3))
+    x[:5] = -0.5
+    x[5:10] = 0.49999
+    g = rng.standard_normal(int(np.prod(n))) + 1j * rng.standard_normal(int(np.prod(n)))
+    f = rng.standard_normal(400) + 1j * rng.standard_normal(400)
+    O.set_threads(threads)
+    try:
+        P = O.Prepsi3d(N, n, m, x)
+        fb = P.B(g)
+        gb = P.BT(f)
+    finally:
+        O.set_threads(1)
+    fr = O.stage_B(N, n, m, x, g)
+    gr = O.stage_BT(N, n, m, x, f)
+    assert np.linalg.norm(fb - fr) / np.linalg.norm(fr) < 1e-13
+    assert np.linalg.norm(gb - gr) / np.linalg.norm(gr) < 1e-13
