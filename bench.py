@@ -267,13 +267,16 @@ def cgnr_leg(args):
 
 def configs_leg(args):
     """The other single-plan workloads, device-resident, precompute excluded:
-    1-D FP64 (N=1024, sigma 2, m=6, M=4096 uniform nodes) and 2-D N=512^2
+    1-D FP64 (N=1024, sigma 2, m=6, M=4096 uniform nodes); 2-D N=512^2
     (sigma 2, M=2^20: half uniform, half Gaussian sd 0.05 at the origin) in
-    FP32 and FP64 with m = 4 and 6.  Per case: ms per trafo+adjoint, nodes/s
+    FP32 and FP64 with m = 4 and 6; and the reference's own benchmark sizes
+    (examples/nfft/nfft_benchomp.c:485-575, sigma 2, m = 4, iid uniform
+    nodes): 1-D N = M = 2^21, 2-D N = 1024^2 with M = 2^20, 3-D N = 128^3
+    with M = 2^21, FP32 and FP64.  Per case: ms per trafo+adjoint, nodes/s
     and the algorithmic HBM bytes of both directions over that time, and the
     relative l2 / l-infinity error of its results: FP32 against the FP64 plan
     on the same inputs (all nodes), FP64 against the FP64 oracle (all nodes
-    in 1-D, 2000 sampled nodes in 2-D)."""
+    in the small 1-D case, 2000 sampled nodes otherwise)."""
     import torch
     import nfft_amd as nf
     import oracle as O
@@ -284,14 +287,27 @@ def configs_leg(args):
     x2 = np.concatenate([r.random((M2 // 2, 2)) - 0.5, 0.05 * r.standard_normal((M2 // 2, 2))])
     x2 = x2 - np.round(x2)
     x2[x2 >= 0.5] -= 1.0
-    cases = [("1d_fp64_m6", [1024], [2048], 6, W.uniform(4096, 1, seed=11), "double")]
+    cases = [("1d_fp64_m6", "1d", [1024], [2048], 6, W.uniform(4096, 1, seed=11), "double")]
     for prec in ("double", "single"):
         for m in (4, 6):
-            cases.append(("2d_%s_m%d" % ("fp32" if prec == "single" else "fp64", m), [512, 512],
-                          [1024, 1024], m, x2, prec))
+            cases.append(("2d_%s_m%d" % ("fp32" if prec == "single" else "fp64", m), "2d512",
+                          [512, 512], [1024, 1024], m, x2, prec))
+    refs = [("ref_1d_2pow21", [1 << 21], [1 << 22], W.uniform(1 << 21, 1, seed=13)),
+            ("ref_2d_1024", [1024, 1024], [2048, 2048], W.uniform(1 << 20, 2, seed=14)),
+            ("ref_3d_128", [128, 128, 128], [256, 256, 256], W.uniform(1 << 21, 3, seed=15))]
+    for prec in ("double", "single"):
+        for name, N, n, x in refs:
+            cases.append(("%s_%s" % (name, "fp32" if prec == "single" else "fp64"), name, N, n, 4, x,
+                          prec))
+    # nodes FP32-representable for both precisions, as the inputs below: an
+    # FP32 plan stores x in float, and at n = 2^22 rounding x to float moves
+    # a node by up to 1/16 cell (a phase error of up to 0.1 rad at the
+    # highest frequency), which is the input's error, not the transform's
+    cases = [c[:5] + (c[5].astype(np.float32).astype(np.float64),) + c[6:] for c in cases]
+    O.set_threads(min(16, len(os.sched_getaffinity(0))))
     out = {}
     res64 = {}
-    for name, N, n, m, x, prec in cases:
+    for name, group, N, n, m, x, prec in cases:
         M = x.shape[0]
         p = nf.Plan(N, M, n=n, m=m, flags=flags, precision=prec)
         p.x[:] = x
@@ -327,8 +343,8 @@ def configs_leg(args):
         a_res = ho.cpu().numpy().astype(np.complex128)
         xr = p.x.astype(np.float64).copy()
         if prec == "double":
-            res64[(len(N), m)] = (t_res, a_res)
-            if len(N) == 1:
+            res64[(group, m)] = (t_res, a_res)
+            if group == "1d":
                 err = {"vs": "FP64 oracle, all nodes",
                        "trafo": err_stats(t_res, O.nfft_trafo(N, n, m, xr, fh_np)),
                        "adjoint": err_stats(a_res, O.nfft_adjoint(N, n, m, xr, f_np))}
@@ -343,13 +359,14 @@ def configs_leg(args):
                 err, _ = oracle_sample_errors(N, n, m, xr, fh_np, f_np, t_res, a_s, 2000, 9)
                 err["vs"] = "FP64 oracle on 2000 sampled nodes"
         else:
-            t64, a64 = res64[(len(N), m)]
+            t64, a64 = res64[(group, m)]
             err = {"vs": "FP64 GPU plan, all nodes", "trafo": err_stats(t_res, t64),
                    "adjoint": err_stats(a_res, a64)}
         out[name]["error"] = err
         log("config %s: %.3f ms per trafo+adjoint, trafo rel l2 %.2e (%s)"
             % (name, dt * 1e3, err["trafo"]["rel_l2"], err["vs"]))
         p.finalize()
+    O.set_threads(1)
     return out
 
 
@@ -554,14 +571,16 @@ def main():
         dom_name, dom_ms, dom_sym = "B interpolation (k_interp_tile, trafo)", b_trafo_ms, "k_interp_tile"
     # HBM bytes per launch of that kernel measured by rocprofv3 FETCH_SIZE /
     # WRITE_SIZE (copy-calibrated) and committed with the profile summaries
+    # (the profile names the exact kernel each stage ran in its run)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_kooshball3d.json")
     if os.path.exists(tpath):
         with open(tpath) as fh_:
             tj = json.load(fh_)
-        for k, v in tj.get("kernels", {}).items():
-            if dom_sym in k:
-                traffic = v["traffic_bytes"]
+        stage = "adjoint_B" if dom_sym == "k_spread_runs" else "trafo_B"
+        kname = tj.get("stage_kernels", {}).get(stage)
+        if kname in tj.get("kernels", {}):
+            traffic = tj["kernels"][kname]["traffic_bytes"]
     achieved = ab["B"] / (dom_ms * 1e-3) / 1e9
     step_bytes = ab["trafo"] + ab["adjoint"]
     taps = (2 * m + 2) ** d * M

@@ -105,3 +105,57 @@ def test_graph_of_many_plans_replays(adjoint):
     finally:
         for p in plans:
             p.finalize()
+
+
+@pytest.mark.gpu
+def test_2d_split_zone_reduction_non_square_core():
+    """2-D n = 1024 x 768: bins of 32 x 24 cells (a core of 768 cells, summed
+    by four workgroups per bin of which the last quarter gets no full row),
+    FP32, against the oracle on a node sample, and batched calls (the zone
+    reduction's batched launch, grid x = bins x parts, y = right-hand side)
+    equal to single calls."""
+    import oracle as O
+    N, n, m = [512, 384], [1024, 768], 4
+    x = W.clustered_2d(200000, seed=12)
+    M = x.shape[0]
+    p = nf.Plan(N, M, n=n, m=m, flags=FLAGS, precision="single")
+    try:
+        p.x[:] = x
+        p.precompute()
+        xr = p.x.astype(np.float64).copy()
+        NN = int(np.prod(N))
+        R = 3
+        fh = rand_dev(R * NN, 31, "single")
+        fi = rand_dev(R * M, 32, "single")
+        t_b = torch.empty(R * M, dtype=torch.complex64, device="cuda")
+        a_b = torch.empty(R * NN, dtype=torch.complex64, device="cuda")
+        p.trafo_batch_device(fh.data_ptr(), t_b.data_ptr(), R)
+        p.adjoint_batch_device(fi.data_ptr(), a_b.data_ptr(), R)
+        p.synchronize()
+        for r in range(R):
+            one = torch.empty(M, dtype=torch.complex64, device="cuda")
+            oneh = torch.empty(NN, dtype=torch.complex64, device="cuda")
+            p.trafo_device(fh[r * NN:].data_ptr(), one.data_ptr())
+            p.adjoint_device(fi[r * M:].data_ptr(), oneh.data_ptr())
+            p.synchronize()
+            assert torch.equal(one, t_b[r * M:(r + 1) * M]), r
+            assert torch.equal(oneh, a_b[r * NN:(r + 1) * NN]), r
+        # right-hand side 0 against the oracle (FP32 inputs, FP64 oracle)
+        fh0 = fh[:NN].cpu().numpy().astype(np.complex128)
+        fi0 = fi[:M].cpu().numpy().astype(np.complex128)
+        t0 = t_b[:M].cpu().numpy().astype(np.complex128)
+        a0 = a_b[:NN].cpu().numpy().astype(np.complex128)
+        # The oracle's D, B, B^T and D^T around numpy's FFT: the oracle's own
+        # FFT is a direct DFT on the 768-point axis (minutes of CPU time at
+        # this size) and equals numpy's to rounding (test_oracle).
+        idx = np.random.default_rng(3).choice(M, size=2000, replace=False)
+        g = np.fft.fftn(O.stage_D(N, n, m, fh0).reshape(n)).ravel()
+        tr = O.stage_B(N, n, m, xr[idx], g)
+        et = float(np.linalg.norm(t0[idx] - tr) / np.linalg.norm(tr))
+        gt = (np.fft.ifftn(O.stage_BT(N, n, m, xr, fi0).reshape(n)) * np.prod(n)).ravel()
+        ar = O.stage_DT(N, n, m, gt)
+        ea = float(np.linalg.norm(a0 - ar) / np.linalg.norm(ar))
+        print("2-D 1024 x 768: trafo (2000 nodes) rel l2 %.2e, adjoint rel l2 %.2e" % (et, ea))
+        assert et < 1e-5 and ea < 1e-5
+    finally:
+        p.finalize()
