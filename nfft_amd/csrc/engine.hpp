@@ -50,8 +50,10 @@ struct BinGeom {
   const int2 *islots;
   // LDS row pitch (cells) of the 3-D lane-per-node interpolation tile:
   // E[2], or padded so that lanes reading windows of different cells of one
-  // plane spread over more LDS banks
+  // plane spread over more LDS banks; ipp its plane pitch (E[1] ip2, or
+  // more)
   int ip2;
+  int ipp;
 };
 
 // Kaiser-Bessel window of one dimension, scaled by 1/phi(0) so that the B
@@ -189,6 +191,7 @@ template <typename R> struct Engine {
   int *scratch_i = nullptr;  // binning scratch
   int *zone_tab = nullptr;   // geom.zmap and geom.zdesc
   int2 *slots = nullptr, *islots = nullptr;  // geom.slots, geom.islots
+  int2 *slots_raw = nullptr;                 // slots in node order before sorting
   long slots_cap = 0, islots_cap = 0, nslots = 0;
   // FP32 3-D interpolation reads the taps of two same-cell nodes once
   // (NFFT_HIP_PAIRS=0: off).  In 2-D the extra window work outweighs the
@@ -200,6 +203,9 @@ template <typename R> struct Engine {
   // wavefronts whose slots hold at most two run the pair code), or 2
   // (NFFT_HIP_SLOT=2)
   int slot_nodes = 4;
+  // each work item's slots ordered by node count, largest first, and dealt
+  // over LDS banks (NFFT_HIP_SLOTSORT=0: node order)
+  int slot_sort = 1;
   // device scratch of the density-weight helpers (features, Pipe-Menon),
   // grown on demand and kept with the plan so repeated calls allocate nothing
   char *aux = nullptr;

@@ -302,6 +302,79 @@ __global__ void k_item_slots(const int4 *__restrict__ subs, long nsub, const int
   islots[i] = make_int2(s0, s1 - s0);
 }
 
+// Orders each work item's interpolation slots by node count, largest first,
+// and deals the slots of one count round-robin over the residue of their
+// window's tile index modulo 32 (the LDS bank pair of its first tap, for
+// rows P2 and planes PP cells apart), in node order within one residue.
+// The interpolation kernel picks its code path per wavefront from the
+// largest count among the wavefront's slots, so sorted slots give
+// wavefronts of one count and few lanes that compute for absent nodes; the
+// dealing makes consecutive lanes read windows that start in different
+// banks (count order alone, lanes reading the windows of scattered cells,
+// measured slower than node order: more LDS bank conflicts).  One
+// workgroup of 256 threads per item; in[] and out[] are the slot arrays
+// before and after, islots[item] = (first slot, slot count).
+template <int Q>
+__global__ __launch_bounds__(256) void k_sort_slots(const int2 *__restrict__ islots,
+                                                    const int2 *__restrict__ in,
+                                                    int2 *__restrict__ out,
+                                                    const int *__restrict__ cell, long M, int P2,
+                                                    int PP) {
+  constexpr int NR = 32;             // residue buckets
+  constexpr int NK = (Q + 1) * NR;   // keys: count * NR + residue
+  __shared__ int hist[NK];           // slots per key in the item
+  __shared__ int run[NK];            // slots per key placed by earlier chunks
+  __shared__ int wcnt[4][NK];        // slots per key in each wavefront's part of a chunk
+  const int2 is = islots[blockIdx.x];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  auto key_of = [&](int2 v) {
+    return v.y * NR + ((cell[v.x] * PP + cell[M + v.x] * P2 + cell[2 * M + v.x]) & (NR - 1));
+  };
+  for (int t = threadIdx.x; t < NK; t += 256) hist[t] = run[t] = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < is.y; k += 256) atomicAdd(&hist[key_of(in[is.x + k])], 1);
+  __syncthreads();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int k0 = 0; k0 < is.y; k0 += 256) {
+    const int k = k0 + threadIdx.x;
+    const bool valid = k < is.y;
+    const int2 v = valid ? in[is.x + k] : make_int2(0, 0);
+    const int key = valid ? key_of(v) : NK;
+    // lanes of this wavefront with the same key (bit by bit)
+    unsigned long long same = __ballot(valid);
+    if (!valid) same = ~same;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const unsigned long long bal = __ballot((key >> b) & 1);
+      same &= ((key >> b) & 1) ? bal : ~bal;
+    }
+    for (int t = threadIdx.x; t < 4 * NK; t += 256) (&wcnt[0][0])[t] = 0;
+    __syncthreads();
+    // the key's lowest lane records the wavefront's count
+    if (valid && (same & below) == 0) wcnt[wv][key] = __popcll(same);
+    __syncthreads();
+    if (valid) {
+      int j = run[key] + __popcll(same & below);
+      for (int w = 0; w < wv; w++) j += wcnt[w][key];
+      const int c = v.y, r = key - c * NR;
+      int pos = 0;
+      for (int cc = Q; cc > c; cc--)
+        for (int rr = 0; rr < NR; rr++) pos += hist[cc * NR + rr];
+      // round j of the dealing: min(h, j) slots of every residue before
+      // it, then those of lower residues that still have a slot
+      for (int rr = 0; rr < NR; rr++) {
+        const int h = hist[c * NR + rr];
+        pos += min(h, j) + (rr < r && h > j ? 1 : 0);
+      }
+      if (pos < is.y) out[is.x + pos] = v;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NK; t += 256)
+      run[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+    __syncthreads();
+  }
+}
+
 __global__ void k_fill_zero_int(int *__restrict__ p, long count) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = 0;
@@ -454,7 +527,9 @@ void bin_nodes(Engine<R> &e) {
     NFFT_HIP_CHECK(R, hipStreamSynchronize(s));
     if (nslots > e.slots_cap) {
       if (e.slots) NFFT_HIP_CHECK(R, hipFree(e.slots));
+      if (e.slots_raw) NFFT_HIP_CHECK(R, hipFree(e.slots_raw));
       NFFT_HIP_CHECK(R, hipMalloc(&e.slots, sizeof(int2) * (size_t)nslots));
+      NFFT_HIP_CHECK(R, hipMalloc(&e.slots_raw, sizeof(int2) * (size_t)nslots));
       e.slots_cap = nslots;
     }
     if (e.nsub > e.islots_cap) {
@@ -462,9 +537,20 @@ void bin_nodes(Engine<R> &e) {
       NFFT_HIP_CHECK(R, hipMalloc(&e.islots, sizeof(int2) * (size_t)e.nsub));
       e.islots_cap = e.nsub;
     }
-    hipLaunchKernelGGL(k_fill_slots, dim3(gm), dim3(256), 0, s, b, sflag, sidx, M, Q, e.slots);
+    // slots in node order, then (e.slot_sort) each item's slots by count
+    // and LDS bank
+    int2 *filled = e.slot_sort ? e.slots_raw : e.slots;
+    hipLaunchKernelGGL(k_fill_slots, dim3(gm), dim3(256), 0, s, b, sflag, sidx, M, Q, filled);
     hipLaunchKernelGGL(k_item_slots, dim3((unsigned)ceil_div(e.nsub, 256)), dim3(256), 0, s,
                        e.subs, e.nsub, sidx, M, nslots, e.islots);
+    if (e.slot_sort) {
+      if (Q == 4)
+        hipLaunchKernelGGL(k_sort_slots<4>, dim3((unsigned)e.nsub), dim3(256), 0, s, e.islots,
+                           filled, e.slots, e.cell, M, e.geom.ip2, e.geom.ipp);
+      else
+        hipLaunchKernelGGL(k_sort_slots<2>, dim3((unsigned)e.nsub), dim3(256), 0, s, e.islots,
+                           filled, e.slots, e.cell, M, e.geom.ip2, e.geom.ipp);
+    }
     e.nslots = nslots;
     e.geom.slots = e.slots;
     e.geom.islots = e.islots;

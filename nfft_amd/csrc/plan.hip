@@ -177,17 +177,28 @@ void Engine<R>::set_bins(const int *bs) {
     geom.E[t] = t < d ? bs[t] + 2 * m + 1 : 1;
   }
   geom.ip2 = geom.E[2];
+  geom.ipp = geom.E[1] * geom.E[2];
   if (d == 3) {
     // interpolation tile rows padded to 4 (mod 8) cells (an 8 x 8 plane of
     // distinct cells then maps to 32 banks of 8 bytes 2-way instead of 4-way
-    // at 17 cells), when the tile still fits; NFFT_HIP_IPITCH=0: unpadded
+    // at 17 cells), when the tile still fits; NFFT_HIP_IPITCH=0: unpadded,
+    // NFFT_HIP_IPITCH=p (p >= E[2]): rows p cells apart; NFFT_HIP_IPLANE=q:
+    // q more cells between planes
     const char *ip = std::getenv("NFFT_HIP_IPITCH");
-    if (!(ip && std::strcmp(ip, "0") == 0)) {
+    const int want = ip ? std::atoi(ip) : -1;
+    int p2 = geom.E[2];
+    if (want >= geom.E[2]) {
+      p2 = want;
+    } else if (want != 0) {
       // (at most 3 cells of padding per row: measured at m = 4, 17 -> 20)
-      int p2 = geom.E[2];
       while (p2 % 8 != 4) p2++;
-      if (p2 - geom.E[2] <= 3 && (size_t)geom.E[0] * geom.E[1] * p2 * sizeof(cplx<R>) <= 64 * 1024)
-        geom.ip2 = p2;
+      if (p2 - geom.E[2] > 3) p2 = geom.E[2];
+    }
+    int pp = geom.E[1] * p2;
+    if (const char *v = std::getenv("NFFT_HIP_IPLANE")) pp += std::max(0, std::atoi(v));
+    if ((size_t)geom.E[0] * pp * sizeof(cplx<R>) <= 64 * 1024) {
+      geom.ip2 = p2;
+      geom.ipp = pp;
     }
   }
   // B^T writes its LDS tiles to scratch and reduces them without atomics
@@ -410,6 +421,7 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   if (const char *v = std::getenv("NFFT_HIP_SYMWIN")) sym_window = std::atoi(v) != 0;
   if (const char *v = std::getenv("NFFT_HIP_PAIRS")) pair_interp = std::atoi(v) != 0;
   if (const char *v = std::getenv("NFFT_HIP_SLOT")) slot_nodes = std::atoi(v) >= 4 ? 4 : 2;
+  if (const char *v = std::getenv("NFFT_HIP_SLOTSORT")) slot_sort = std::atoi(v) != 0;
   // float: plain FP32 read-modify-write by wavefronts that own disjoint
   // cells, same-cell runs summed in registers (3-D: two wavefronts per tile
   // split by row parity; 2-D: one wavefront per tile; 1-D: private tiles);
@@ -616,6 +628,7 @@ void Engine<R>::destroy() {
   hipFree(bgrid);
   hipFree(btiles);
   hipFree(slots);
+  hipFree(slots_raw);
   hipFree(islots);
   for (auto &e : ev)
     if (e) hipEventDestroy(e);

@@ -232,14 +232,14 @@ __device__ __forceinline__ void node_window_sym(const int *__restrict__ cell,
 }
 
 // Stages the grid region of a 3-D work item (E0 x E1 x E2 cells from
-// lo - m, periodic) into LDS, rows gg.ip2 cells apart.  Each thread keeps one position along the last
+// lo - m, periodic) into LDS, rows gg.ip2 and planes gg.ipp cells apart.  Each thread keeps one position along the last
 // dimension and walks whole rows, so per element it does one table lookup
 // per leading dimension instead of divisions and modulos.
 template <typename R>
 __device__ __forceinline__ void load_region_3d(const cplx<R> *__restrict__ g, cplx<R> *tile,
                                                const int (&lo)[3], const BinGeom &gg,
                                                long *rowoff0, long *rowoff1) {
-  const int E0 = gg.E[0], E1 = gg.E[1], E2 = gg.E[2], P2 = gg.ip2;
+  const int E0 = gg.E[0], E1 = gg.E[1], E2 = gg.E[2], P2 = gg.ip2, PP = gg.ipp;
   auto wrap = [](int v, int n) {
     v %= n;
     return v < 0 ? v + n : v;
@@ -264,7 +264,7 @@ __device__ __forceinline__ void load_region_3d(const cplx<R> *__restrict__ g, cp
       for (int u = 0; u < B; u++) {
         dst[u] = -1;
         if (e0 < E0) {
-          dst[u] = (e0 * E1 + e1) * P2 + e2;
+          dst[u] = e0 * PP + e1 * P2 + e2;
           v[u] = g[rowoff0[e0] + rowoff1[e1] + col];
           e1 += rows_per_pass;
           while (e1 >= E1) {
@@ -280,7 +280,7 @@ __device__ __forceinline__ void load_region_3d(const cplx<R> *__restrict__ g, cp
   }
 }
 
-// Interpolation of one slot of up to four nodes of one cell (3-D, FP32, the
+// Interpolation of one slot of up to Q nodes of one cell (3-D, FP32, the
 // shared symmetric window table): every grid tap of the slot's window is
 // read once from LDS for all its nodes.  The dimension-1 and -2 windows of
 // the nodes stay in registers; the dimension-0 weights are evaluated row
@@ -288,19 +288,19 @@ __device__ __forceinline__ void load_region_3d(const cplx<R> *__restrict__ g, cp
 // parts), which keeps the kernel at three wavefronts per SIMD.  Nodes past
 // the slot's count repeat its last node and are not written; a lane with
 // live = false writes nothing.
-template <int W>
-__device__ __forceinline__ void interp_quad_3d(const float2 *tile, float2 *__restrict__ fout,
+template <int W, int Q>
+__device__ __forceinline__ void interp_slot_3d(const float2 *tile, float2 *__restrict__ fout,
                                                const int *__restrict__ perm,
                                                const int *__restrict__ cell,
                                                const float *__restrict__ frac, const BinGeom &gg,
                                                const WinParams<float> &wp, const int (&lo)[3],
-                                               int E1, int E2, int2 pr, bool live) {
+                                               int PP, int E2, int2 pr, bool live) {
   constexpr int KE = kPolyDeg - (kPolyDeg % 2);      // highest even power
   constexpr int KO = kPolyDeg - 1 + (kPolyDeg % 2);  // highest odd power
   const long M = gg.M;
-  long pn[4];
+  long pn[Q];
 #pragma unroll
-  for (int q = 0; q < 4; q++) pn[q] = (long)pr.x + min(q, pr.y - 1);
+  for (int q = 0; q < Q; q++) pn[q] = (long)pr.x + min(q, pr.y - 1);
   // the symmetric tap pair (w, W-1-w) at tt = frac - 1/2
   auto tap_pair = [&](int w, float tt, float &lo_v, float &hi_v) {
     const float *pc = wp.poly + w * kPolyCoef;
@@ -314,9 +314,9 @@ __device__ __forceinline__ void interp_quad_3d(const float2 *tile, float2 *__res
     lo_v = fmaf(tt, o, e);
     hi_v = fmaf(-tt, o, e);
   };
-  float p1[4][W], p2[4][W], t0[4];
+  float p1[Q][W], p2[Q][W], t0[Q];
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
+  for (int q = 0; q < Q; q++) {
     t0[q] = frac[pn[q]] - 0.5f;
     const float t1 = frac[M + pn[q]] - 0.5f;
     const float t2 = frac[2 * M + pn[q]] - 0.5f;
@@ -326,39 +326,39 @@ __device__ __forceinline__ void interp_quad_3d(const float2 *tile, float2 *__res
       tap_pair(w, t2, p2[q][w], p2[q][W - 1 - w]);
     }
   }
-  const float2 *base = tile + ((cell[pr.x] - lo[0]) * E1 + (cell[M + pr.x] - lo[1])) * E2 +
+  const float2 *base = tile + (cell[pr.x] - lo[0]) * PP + (cell[M + pr.x] - lo[1]) * E2 +
                        (cell[2 * M + pr.x] - lo[2]);
-  float ar[4] = {0.f, 0.f, 0.f, 0.f}, ai[4] = {0.f, 0.f, 0.f, 0.f};
+  float ar[Q] = {}, ai[Q] = {};
   for (int ip = 0; ip < W / 2; ip++) {
-    float w0[2][4];
+    float w0[2][Q];
 #pragma unroll
-    for (int q = 0; q < 4; q++) tap_pair(ip, t0[q], w0[0][q], w0[1][q]);
+    for (int q = 0; q < Q; q++) tap_pair(ip, t0[q], w0[0][q], w0[1][q]);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const int i0 = h == 0 ? ip : W - 1 - ip;
-      float qr[4] = {0.f, 0.f, 0.f, 0.f}, qi[4] = {0.f, 0.f, 0.f, 0.f};
+      float qr[Q] = {}, qi[Q] = {};
       for (int i1 = 0; i1 < W; i1++) {
-        const float2 *row = base + (i0 * E1 + i1) * E2;
+        const float2 *row = base + i0 * PP + i1 * E2;
         // the row's taps are all requested before the first is used
         float2 v[W];
 #pragma unroll
         for (int w = 0; w < W; w++) v[w] = row[w];
-        float sr[4] = {0.f, 0.f, 0.f, 0.f}, si[4] = {0.f, 0.f, 0.f, 0.f};
+        float sr[Q] = {}, si[Q] = {};
 #pragma unroll
         for (int w = 0; w < W; w++)
 #pragma unroll
-          for (int q = 0; q < 4; q++) {
+          for (int q = 0; q < Q; q++) {
             sr[q] = fmaf(p2[q][w], v[w].x, sr[q]);
             si[q] = fmaf(p2[q][w], v[w].y, si[q]);
           }
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < Q; q++) {
           qr[q] = fmaf(p1[q][i1], sr[q], qr[q]);
           qi[q] = fmaf(p1[q][i1], si[q], qi[q]);
         }
       }
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
+      for (int q = 0; q < Q; q++) {
         ar[q] = fmaf(w0[h][q], qr[q], ar[q]);
         ai[q] = fmaf(w0[h][q], qi[q], ai[q]);
       }
@@ -366,7 +366,7 @@ __device__ __forceinline__ void interp_quad_3d(const float2 *tile, float2 *__res
   }
   if (live) {
 #pragma unroll
-    for (int q = 0; q < 4; q++)
+    for (int q = 0; q < Q; q++)
       if (q < pr.y) fout[perm[pr.x + q]] = float2{ar[q], ai[q]};
   }
 }
@@ -374,9 +374,11 @@ __device__ __forceinline__ void interp_quad_3d(const float2 *tile, float2 *__res
 // PAIR (D = 2, 3): threads take the work item's slots (gg.islots / gg.slots)
 // instead of single nodes; a slot of two nodes of one cell reads each grid
 // tap once from LDS for both, halving the LDS bytes per node of dense runs.
-// With SYM in 3-D a slot may hold up to four nodes: a wavefront whose slots
-// hold at most two runs the pair code, otherwise the quad code
-// (interp_quad_3d), which reads each tap once for four nodes.
+// With SYM in 3-D a slot may hold up to four nodes and the largest count
+// among a wavefront's slots picks its code: the pair code for two, the
+// slot code (interp_slot_3d) for one, three or four nodes, which reads each
+// tap once for all of them.  Precompute orders an item's slots by count, so
+// that most wavefronts hold slots of one count.
 template <typename R, int D, int W, bool SYM = false, bool PAIR = false>
 __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__ g,
                                                      cplx<R> *__restrict__ fout,
@@ -396,6 +398,9 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
   int tot = 1;
 #pragma unroll
   for (int t = 0; t < D; t++) tot *= gg.E[t];
+  // next 64-slot chunk of the work item (PAIR)
+  __shared__ int next_chunk;
+  if (threadIdx.x == 0) next_chunk = 0;
   if constexpr (D == 3) {
     __shared__ long rowoff[2][64];
     load_region_3d<R>(g, tile, lo, gg, rowoff[0], rowoff[1]);
@@ -403,13 +408,21 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
     for (int q = threadIdx.x; q < tot; q += blockDim.x) tile[q] = g[region_to_grid<D>(q, lo, gg)];
   }
   const int E1 = D >= 2 ? gg.E[1] : 1;
-  // the tile's row pitch (load_region_3d)
+  // the tile's row and plane pitches (load_region_3d)
   const int E2 = D == 3 ? gg.ip2 : 1;
+  const int PP = D == 3 ? gg.ipp : 0;
   __syncthreads();
   if constexpr (PAIR && (D == 2 || D == 3)) {
     const int2 is = gg.islots[blockIdx.x];
-    for (int k0 = 0; k0 < is.y; k0 += blockDim.x) {
-      const int k = k0 + threadIdx.x;
+    // wavefronts take the item's 64-slot chunks in turn as they finish
+    // their last, so a wavefront with light chunks does not idle while
+    // another works through heavy ones
+    for (;;) {
+      int ch = 0;
+      if ((threadIdx.x & 63) == 0) ch = atomicAdd(&next_chunk, 1);
+      ch = __shfl(ch, 0, 64);
+      if (ch * 64 >= is.y) break;
+      const int k = ch * 64 + (threadIdx.x & 63);
       // (first node, count) of this lane's slot; lanes past the item's
       // slots take its first slot and write nothing
       const int2 pr = gg.slots[is.x + (k < is.y ? k : 0)];
@@ -419,8 +432,14 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
         int wmax = live ? pr.y : 0;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
-        if (wmax > 2) {
-          interp_quad_3d<W>(tile, fout, perm, cell, frac, gg, wp, lo, E1, E2, pr, live);
+        if (wmax == 0) continue;
+        if (wmax != 2) {
+          if (wmax == 4)
+            interp_slot_3d<W, 4>(tile, fout, perm, cell, frac, gg, wp, lo, PP, E2, pr, live);
+          else if (wmax == 3)
+            interp_slot_3d<W, 3>(tile, fout, perm, cell, frac, gg, wp, lo, PP, E2, pr, live);
+          else
+            interp_slot_3d<W, 1>(tile, fout, perm, cell, frac, gg, wp, lo, PP, E2, pr, live);
           continue;
         }
       }
@@ -458,12 +477,12 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
           bi += pb_[0][i0] * sbi;
         }
       } else {
-        const cplx<R> *base = tile + ((c[0] - lo[0]) * E1 + (c[1] - lo[1])) * E2 + (c[2] - lo[2]);
+        const cplx<R> *base = tile + (c[0] - lo[0]) * PP + (c[1] - lo[1]) * E2 + (c[2] - lo[2]);
         for (int i0 = 0; i0 < W; i0++) {
           R qar = 0, qai = 0, qbr = 0, qbi = 0;
 #pragma unroll
           for (int i1 = 0; i1 < W; i1++) {
-            const cplx<R> *row = base + (i0 * E1 + i1) * E2;
+            const cplx<R> *row = base + i0 * PP + i1 * E2;
             // the row's taps are all requested before the first is used, so
             // a wavefront keeps several LDS reads in flight (short windows
             // read in the loop: the compiler would hoist every row)
@@ -534,12 +553,12 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
         ai += psi[0][i0] * si;
       }
     } else {
-      const cplx<R> *base = tile + ((c[0] - lo[0]) * E1 + (c[1] - lo[1])) * E2 + (c[2] - lo[2]);
+      const cplx<R> *base = tile + (c[0] - lo[0]) * PP + (c[1] - lo[1]) * E2 + (c[2] - lo[2]);
       for (int i0 = 0; i0 < W; i0++) {
         R qr = 0, qi = 0;
 #pragma unroll
         for (int i1 = 0; i1 < W; i1++) {
-          const cplx<R> *row = base + (i0 * E1 + i1) * E2;
+          const cplx<R> *row = base + i0 * PP + i1 * E2;
           cplx<R> v[W];
           if constexpr (W >= 8) {
 #pragma unroll
@@ -1656,8 +1675,8 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
                                           bool spread, int variant, size_t &lds, int &threads,
                                           bool shared_poly = false) {
   size_t bytes = tile_bytes<R>(gg);
-  // the lane-per-node 3-D interpolation tile has rows gg.ip2 cells apart
-  if (!spread && variant == 0 && gg.d == 3) bytes = bytes / gg.E[2] * gg.ip2;
+  // the lane-per-node 3-D interpolation tile has planes gg.ipp cells apart
+  if (!spread && variant == 0 && gg.d == 3) bytes = (size_t)gg.E[0] * gg.ipp * sizeof(cplx<R>);
   const bool tile = bytes <= kTileBudget;
   lds = tile ? bytes : 0;
   threads = 256;

@@ -299,18 +299,23 @@ def test_four_dimensional(precision, m, N, n):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("pairs,slot", [("0", "2"), ("1", "2"), ("1", "4")])
-@pytest.mark.parametrize("pitch", ["0", "1"])
-def test_interp_pairs_on_and_off(pairs, slot, pitch, monkeypatch):
+@pytest.mark.parametrize("pairs,slot,order", [("0", "2", "1"), ("1", "2", "1"), ("1", "4", "1"),
+                                              ("1", "4", "0")])
+@pytest.mark.parametrize("pitch,plane", [("0", "0"), ("1", "0"), ("22", "3")])
+def test_interp_pairs_on_and_off(pairs, slot, order, pitch, plane, monkeypatch):
     """3-D FP32 interpolation without same-cell slots, with slots of up to two
     and of up to four nodes (dense kooshball centre: most nodes share their
-    cell; the outer shell mixes cells of one to three nodes, so both the
-    pair and the quad code run, and quad slots of three) against the oracle,
-    with the interpolation tile's rows at their natural length and padded
-    (NFFT_HIP_IPITCH)."""
+    cell; the outer shell mixes cells of one to three nodes, so the single,
+    pair, triple and quad code all run), each item's slots in count and bank
+    order or in node order (NFFT_HIP_SLOTSORT), against the oracle, with the
+    interpolation tile's rows at their natural length, padded to the default
+    pitch, and at 22 cells with 3 more cells between planes
+    (NFFT_HIP_IPITCH, NFFT_HIP_IPLANE)."""
     monkeypatch.setenv("NFFT_HIP_PAIRS", pairs)
     monkeypatch.setenv("NFFT_HIP_SLOT", slot)
+    monkeypatch.setenv("NFFT_HIP_SLOTSORT", order)
     monkeypatch.setenv("NFFT_HIP_IPITCH", pitch)
+    monkeypatch.setenv("NFFT_HIP_IPLANE", plane)
     N = [16, 16, 16]
     x = W.kooshball(96, 64, seed=5)
     fh = W.complex_normal(int(np.prod(N)), seed=81)
