@@ -54,6 +54,13 @@ struct BinGeom {
   // more)
   int ip2;
   int ipp;
+  // 3-D run-merging spreading (spread_runs.hpp): the LDS tile's plane pitch
+  // (cells; E[1] E[2] or padded) and the lane table, sp_lane[lane + 64 slot]
+  // = the (dimension-0 tap, dimension-2 pair) index the lane owns (255:
+  // none); sp_gc = cells per run along dimension 2 (2 for an even bin edge)
+  int sp_pp;
+  int sp_gc;
+  unsigned char sp_lane[192];
 };
 
 // Kaiser-Bessel window of one dimension, scaled by 1/phi(0) so that the B
@@ -65,14 +72,16 @@ template <typename R> struct WinParams {
   // Per dimension t and tap w, the scaled window psi(frac + m - w) as a
   // degree kPolyDeg polynomial in (frac - 1/2), coefficients lowest first at
   // poly[(t * (2m+2) + w) * kPolyCoef + k]; fitted on the host in long double
-  // (Chebyshev interpolation, |error| < 1e-9 of the peak for m <= 8).
+  // (Chebyshev interpolation; at degree 8 the largest error is 1.7e-8 of the
+  // peak at m = 4, sigma = 2, 1.9e-7 at m = 2 and below 5e-8 for m >= 3,
+  // against FP32's 6e-8 rounding of values near the peak).
   // The table is mirror-symmetric: tap 2m+1-w at t is tap w at -t, so its
   // coefficients are those of tap w with odd powers negated.
   const R *poly;
   // every dimension has the same sigma_t, hence the same tap polynomials
   int poly_shared;
 };
-constexpr int kPolyDeg = 10;
+constexpr int kPolyDeg = 8;
 constexpr int kPolyCoef = kPolyDeg + 1;
 
 // Twiddles and radix schedule of the FFT along one grid dimension.

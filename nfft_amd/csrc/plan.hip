@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 
 #include "../../include/nfft3.h"
@@ -161,6 +162,80 @@ static bool fft_radices(long n, std::vector<int> &out) {
   return out.size() <= 20;
 }
 
+// Lane table and plane pitch of the 3-D run-merging spreading kernel for W
+// taps, runs of gc cells along dimension 2 and an E0 x E1 x E2 tile.  Its
+// lanes own (dimension-0 tap, dimension-2 pair) cells and one flush
+// instruction updates one dimension-1 row of every lane's cells, at tile
+// offsets pa pp + pb from a common base.  Lanes whose 8-byte cells share an
+// LDS bank serialise, so the table and the padding of the plane pitch (up
+// to what keeps the same number of tiles per CU in 160 KB of LDS) minimise
+// the bank collisions of those offsets: per 16-lane group the largest count
+// of one residue mod 16, plus per 32-lane group of one residue mod 32
+// (8-byte units; either grouping may be the one the LDS serves 64-bit
+// accesses in).  Greedy placement, then pairwise swaps from a fixed seed;
+// deterministic, cached per geometry.
+static void spread_lane_table(int W, int gc, int E0, int E1, int E2, int &pp_out,
+                              unsigned char *table) {
+  static std::mutex mu;
+  static std::map<std::array<int, 5>, std::pair<int, std::vector<unsigned char>>> cache;
+  const std::array<int, 5> key{W, gc, E0, E1, E2};
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(key);
+  if (it == cache.end()) {
+    const int hw = (W + gc) / 2, pairs = W * hw, slots = (pairs + 63) / 64, nl = 64 * slots;
+    const int plane = E1 * E2;
+    const long lds = 160 * 1024 / 8;  // cells
+    const long per_cu = lds / ((long)E0 * plane);
+    const int pmax = per_cu > 0 ? (int)std::min<long>(32, lds / per_cu / E0 - plane) : 0;
+    auto cost = [&](const std::vector<int> &t, int pp) {
+      int c = 0;
+      for (int g = 0; g < nl; g += 16) {
+        int h[16] = {0}, mx = 0;
+        for (int l = g; l < g + 16; l++)
+          if (t[l] < pairs) mx = std::max(mx, ++h[((t[l] / hw) * pp + 2 * (t[l] % hw)) & 15]);
+        c += mx;
+      }
+      for (int g = 0; g < nl; g += 32) {
+        int h[32] = {0}, mx = 0;
+        for (int l = g; l < g + 32; l++)
+          if (t[l] < pairs) mx = std::max(mx, ++h[((t[l] / hw) * pp + 2 * (t[l] % hw)) & 31]);
+        c += mx;
+      }
+      return c;
+    };
+    int best_pp = plane, best_c = 1 << 30;
+    std::vector<int> best;
+    for (int pad = 0; pad <= std::max(0, pmax); pad++) {
+      const int pp = plane + pad;
+      std::vector<int> t(nl, 255);
+      for (int q = 0; q < pairs; q++) t[q] = q;
+      int c = cost(t, pp);
+      unsigned s = 12345u;
+      for (int it2 = 0; it2 < 3000 && c > 0; it2++) {
+        s = s * 1664525u + 1013904223u;
+        const int i = (int)((s >> 8) % (unsigned)nl);
+        s = s * 1664525u + 1013904223u;
+        const int j = (int)((s >> 8) % (unsigned)nl);
+        if (t[i] == t[j]) continue;
+        std::swap(t[i], t[j]);
+        const int c2 = cost(t, pp);
+        if (c2 <= c) c = c2;
+        else std::swap(t[i], t[j]);
+      }
+      if (c < best_c) {
+        best_c = c;
+        best_pp = pp;
+        best = t;
+      }
+    }
+    std::vector<unsigned char> tab(192, 255);
+    for (int l = 0; l < nl && l < 192; l++) tab[l] = (unsigned char)best[l];
+    it = cache.emplace(key, std::make_pair(best_pp, tab)).first;
+  }
+  pp_out = it->second.first;
+  std::memcpy(table, it->second.second.data(), 192);
+}
+
 // Bin edges bs[t] (and with them the LDS tile edge bs + 2m + 1): the
 // geometry shared by the node binning and the B / B^T kernels, whether B^T
 // reduces its tiles through scratch, and the zone tables of that reduction.
@@ -200,6 +275,16 @@ void Engine<R>::set_bins(const int *bs) {
       geom.ip2 = p2;
       geom.ipp = pp;
     }
+  }
+  // the run-merging spreading kernel's lane table (3-D, 2m + 2 taps; runs
+  // of two cells along dimension 2 for an even bin edge there)
+  geom.sp_gc = 1;
+  geom.sp_pp = geom.E[1] * geom.E[2];
+  std::memset(geom.sp_lane, 255, sizeof(geom.sp_lane));
+  if (d == 3 && 2 * m + 2 <= 20) {
+    geom.sp_gc = geom.bs[2] % 2 == 0 ? 2 : 1;
+    spread_lane_table(2 * m + 2, geom.sp_gc, geom.E[0], geom.E[1], geom.E[2], geom.sp_pp,
+                      geom.sp_lane);
   }
   // B^T writes its LDS tiles to scratch and reduces them without atomics
   // when the tile fits LDS and never wraps onto itself (E_t <= n_t)

@@ -70,7 +70,13 @@ __device__ __forceinline__ void rows_bcast(float2 (&acc)[RW][SLOTS][2], float br
 // clearing; no other stream of the wavefront writes those cells meanwhile
 // and the other wavefronts own other rows, so the load-store pair is
 // race-free.
-template <int W, int NW, int G, int S, bool RS = false>
+// GC: a run is the nodes of GC adjacent cells along dimension 2 (same
+// dimension-0 and dimension-1 cell, dimension-2 cells 2k .. 2k+GC-1 of the
+// bin), whose windows fit W + GC - 1 dimension-2 taps; a node at dimension-2
+// offset delta in its group takes its taps from lane tap - delta.  With
+// GC = 2 a run holds the nodes of two cells, so runs are about twice as long
+// and flush half as often (needs an even bin edge in dimension 2).
+template <int W, int NW, int G, int S, bool RS = false, int GC = 1>
 __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restrict__ fin,
                                                          float2 *__restrict__ g,
                                                          const int *__restrict__ perm,
@@ -80,15 +86,22 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
                                                          BinGeom gg, WinParams<float> wp,
                                                          float2 *__restrict__ tiles) {
   static_assert(W % 2 == 0 && W % NW == 0 && 3 * W <= 64, "W = 2m+2 taps, rows split evenly");
-  constexpr int HW = W / 2;
+  static_assert(GC == 1 || GC == 2, "runs of one or two cells");
+  constexpr int HW = (W + GC) / 2;  // dimension-2 pairs covering W + GC - 1 taps
   constexpr int PAIRS = W * HW;
   constexpr int SLOTS = (PAIRS + 63) / 64;
   constexpr int RW = W / NW;
+  // (Evaluating one node per pass with its coordinates and value taken by
+  // readlane, every 16-lane row holding the dimension-1 taps so that the
+  // FMAs read the row weights in place, replaced 4.5 of the 7.5 ds_bpermute
+  // per node and wavefront but measured slower: 4.20 -> 5.35 ms.)
   constexpr int NP = 3 * W <= 32 ? 2 : 1;  // nodes per window pass
   constexpr int NPASS = G / NP;
+  constexpr int ZL = 3 * W;                 // a window lane that holds 0
   static_assert(G % NP == 0 && 64 % G == 0, "groups tile the 64-node chunk");
   static_assert(RW <= 16, "a wavefront's rows fit one 16-lane DPP row");
   static_assert(!RS || S == 1, "run load-store needs one stream per wavefront");
+  static_assert(SLOTS <= 3, "lane table covers three slots");
   // right-hand side blockIdx.y of a batched launch
   fin += (long)blockIdx.y * gg.bs_f;
   g += (long)blockIdx.y * gg.bs_g;
@@ -98,42 +111,57 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
   const int4 sp = subs[blockIdx.x];
   int lo[3];
   bin_corner<3>(sp.x, gg, lo);
-  const int E1 = gg.E[1], E2 = gg.E[2];
-  const int plane = E1 * E2;
+  // LDS layout: cell (e0, e1, e2) at e0 PP + e1 E2 + e2, the plane pitch PP
+  // padded so that the lanes of one row update spread over the LDS banks
+  const int E2 = gg.E[2];
+  const int PP = gg.sp_pp;
+  const int plane = gg.E[1] * E2;
   const int tot = gg.E[0] * plane;
-  for (int q = threadIdx.x; q < tot; q += blockDim.x) tile[q] = float2{0.f, 0.f};
+  const int ptot = gg.E[0] * PP;
+  for (int q = threadIdx.x; q < ptot; q += blockDim.x) tile[q] = float2{0.f, 0.f};
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
-  // (dimension-1 tap, dimension-2 pair) slots of this lane
+  // (dimension-0 tap, dimension-2 pair) slots of this lane from the plan's
+  // lane table; c1[sl]: the pair's second cell lies inside the run's taps
   int off[SLOTS];
-  bool pv[SLOTS];
-  int adr_u[NP][SLOTS], adr_v0[NP][SLOTS], adr_v1[NP][SLOTS];
+  bool pv[SLOTS], c1[SLOTS];
+  // (adr_v*s: the node one cell further along dimension 2 in its group,
+  // taps shifted by one; separate arrays, so that choosing between them
+  // stays a select and never becomes a dynamically indexed private array)
+  int adr_u[NP][SLOTS], adr_v0[NP][SLOTS], adr_v1[NP][SLOTS], adr_v0s[NP][SLOTS],
+      adr_v1s[NP][SLOTS];
 #pragma unroll
   for (int sl = 0; sl < SLOTS; sl++) {
-    const int q = lane + 64 * sl;
+    const int q = gg.sp_lane[lane + 64 * sl];
     pv[sl] = q < PAIRS;
     const int pa = pv[sl] ? q / HW : 0;
     const int pb = pv[sl] ? 2 * (q - (q / HW) * HW) : 0;
-    off[sl] = pa * E2 + pb;
+    c1[sl] = pv[sl] && pb + 1 < W + GC - 1;
+    off[sl] = pa * PP + pb;
 #pragma unroll
     for (int h = 0; h < NP; h++) {
-      adr_u[h][sl] = 4 * (h * 32 + W + pa);
-      adr_v0[h][sl] = 4 * (h * 32 + 2 * W + pb);
-      adr_v1[h][sl] = 4 * (h * 32 + 2 * W + pb + 1);
+      adr_u[h][sl] = 4 * (h * 32 + pa);
+      auto tap_adr = [&](int t) { return 4 * (h * 32 + (t >= 0 && t < W ? 2 * W + t : ZL)); };
+      adr_v0[h][sl] = tap_adr(pb);
+      adr_v1[h][sl] = tap_adr(pb + 1);
+      adr_v0s[h][sl] = tap_adr(pb - 1);
+      adr_v1s[h][sl] = tap_adr(pb);
     }
   }
-  // window tap evaluated by this lane in a pass: node half h, dimension wt, tap ww
+  // window tap evaluated by this lane in a pass: node half h, dimension wt,
+  // tap ww; lanes without a tap evaluate 0 (all coefficients 0)
   const int h = NP == 2 ? lane >> 5 : 0;
   const int li = NP == 2 ? lane & 31 : lane;
-  const int wt = li / W < 3 ? li / W : 2;
-  const int ww = li - (li / W) * W;
+  const bool tap_lane = li < 3 * W;
+  const int wt = tap_lane ? li / W : 0;
+  const int ww = tap_lane ? li - wt * W : 0;
   float coef[kPolyCoef];
   {
     const float *pc = wp.poly + (wt * W + ww) * kPolyCoef;
 #pragma unroll
-    for (int k = 0; k < kPolyCoef; k++) coef[k] = pc[k];
+    for (int k = 0; k < kPolyCoef; k++) coef[k] = tap_lane ? pc[k] : 0.f;
   }
 
   // streams: node ranges [sb, sb + sl) of the work item
@@ -154,15 +182,19 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
     for (int j = 0; j < RW; j++)
 #pragma unroll
       for (int sl = 0; sl < SLOTS; sl++) acc[st][j][sl][0] = acc[st][j][sl][1] = float2{0.f, 0.f};
-  // current run per stream: packed key (tile corner | 4 r0 << 20, r0 the
-  // first of the node's window rows that this wavefront owns), from the
-  // stream's first node on (a stream without nodes holds the item's first
-  // node's corner and only ever adds zeros there)
+  // current run per stream: packed key (group corner | delta << 20 | r0 <<
+  // 21, r0 the first of the node's dimension-1 window rows that this
+  // wavefront owns, delta the node's dimension-2 offset in its group; runs
+  // compare without delta), from the stream's first node on (a stream
+  // without nodes holds the item's first node's corner and only ever adds
+  // zeros there)
+  constexpr int kDelta = 1 << 20;
   int cur[S];
+  auto cell_at = [&](int st, int sl, int j) {
+    return tile + (cur[st] & 0xfffff) + ((cur[st] >> 21) + j * NW) * E2 + off[sl];
+  };
   // the run's sums into the tile (read-modify-write), accumulators cleared
   auto flush = [&](int st) {
-    const int corner = cur[st] & 0xfffff;
-    const int r0 = cur[st] >> 22;
     // all reads, then all writes: the rows are distinct cells
 #pragma unroll
     for (int sl = 0; sl < SLOTS; sl++) {
@@ -170,15 +202,16 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
         float2 v[RW][2];
 #pragma unroll
         for (int j = 0; j < RW; j++) {
-          const float2 *t = tile + corner + (r0 + j * NW) * plane + off[sl];
+          const float2 *t = cell_at(st, sl, j);
           v[j][0] = t[0];
-          v[j][1] = t[1];
+          v[j][1] = c1[sl] ? t[1] : float2{0.f, 0.f};
         }
 #pragma unroll
         for (int j = 0; j < RW; j++) {
-          float2 *t = tile + corner + (r0 + j * NW) * plane + off[sl];
+          float2 *t = cell_at(st, sl, j);
           t[0] = float2{v[j][0].x + acc[st][j][sl][0].x, v[j][0].y + acc[st][j][sl][0].y};
-          t[1] = float2{v[j][1].x + acc[st][j][sl][1].x, v[j][1].y + acc[st][j][sl][1].y};
+          if (c1[sl])
+            t[1] = float2{v[j][1].x + acc[st][j][sl][1].x, v[j][1].y + acc[st][j][sl][1].y};
         }
       }
 #pragma unroll
@@ -187,18 +220,15 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
   };
   // RS: the run's sums are the tile's values; store them, and load the next
   // run's (after the store in program order: the runs' cells overlap)
-  auto run_cells = [&](int st, int sl, int j) {
-    return tile + (cur[st] & 0xfffff) + ((cur[st] >> 22) + j * NW) * plane + off[sl];
-  };
   auto store = [&](int st) {
 #pragma unroll
     for (int sl = 0; sl < SLOTS; sl++)
       if (pv[sl]) {
 #pragma unroll
         for (int j = 0; j < RW; j++) {
-          float2 *t = run_cells(st, sl, j);
+          float2 *t = cell_at(st, sl, j);
           t[0] = acc[st][j][sl][0];
-          t[1] = acc[st][j][sl][1];
+          if (c1[sl]) t[1] = acc[st][j][sl][1];
         }
       }
   };
@@ -208,9 +238,9 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
       if (pv[sl]) {
 #pragma unroll
         for (int j = 0; j < RW; j++) {
-          const float2 *t = run_cells(st, sl, j);
+          const float2 *t = cell_at(st, sl, j);
           acc[st][j][sl][0] = t[0];
-          acc[st][j][sl][1] = t[1];
+          acc[st][j][sl][1] = c1[sl] ? t[1] : float2{0.f, 0.f};
         }
       }
   };
@@ -251,9 +281,11 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
       a[t] = nc[st][t] - lo[t];
       cfr[st][t] = nfr[st][t] - 0.5f;
     }
-    // the corner determines r0, so keys of one wavefront compare as corners
-    const int r0 = ((wv - a[0]) % NW + NW) % NW;
-    ckey[st] = ((a[0] * E1 + a[1]) * E2 + a[2]) | (r0 << 22);
+    // the dimension-1 cell determines r0, so keys of one wavefront compare
+    // as corners
+    const int r0 = ((wv - a[1]) % NW + NW) % NW;
+    const int dl = GC == 2 ? (a[2] & 1) : 0;
+    ckey[st] = (a[0] * PP + a[1] * E2 + a[2] - dl) | (dl * kDelta) | (r0 << 21);
     cf[st] = first + lane < sn[st] ? nf[st] : float2{0.f, 0.f};
   };
   int longest = 0;
@@ -263,7 +295,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
     nperm[st] = perm[pos(st, 64)];
     nnperm[st] = 0;
     convert(st, 0);
-    cur[st] = __builtin_amdgcn_readfirstlane(ckey[st]);
+    cur[st] = __builtin_amdgcn_readfirstlane(ckey[st]) & ~kDelta;
     longest = max(longest, sn[st]);
   }
   for (int first = 0; first < longest; first += 64) {
@@ -283,8 +315,11 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
       // the stream's count: unused)
       float xr[S][NPASS], xi[S][NPASS];
       float w0[S][G][SLOTS], w1[S][G][SLOTS];
+      int gkey[S][G];
 #pragma unroll
       for (int st = 0; st < S; st++) {
+#pragma unroll
+        for (int k = 0; k < G; k++) gkey[st][k] = __builtin_amdgcn_readlane(ckey[st], base + k);
         float mine[NPASS];
 #pragma unroll
         for (int p = 0; p < NPASS; p++) {
@@ -305,27 +340,28 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
 #pragma unroll
         for (int k = 0; k < G; k++) {
           const int p = k / NP, hk = k % NP;
+          const bool dl = GC == 2 && (gkey[st][k] & kDelta);
 #pragma unroll
           for (int sl = 0; sl < SLOTS; sl++) {
             const float u = bperm_f(adr_u[hk][sl], mine[p]);
-            w0[st][k][sl] = u * bperm_f(adr_v0[hk][sl], mine[p]);
-            w1[st][k][sl] = u * bperm_f(adr_v1[hk][sl], mine[p]);
+            const int a0 = dl ? adr_v0s[hk][sl] : adr_v0[hk][sl];
+            const int a1 = dl ? adr_v1s[hk][sl] : adr_v1[hk][sl];
+            w0[st][k][sl] = u * bperm_f(a0, mine[p]);
+            w1[st][k][sl] = u * bperm_f(a1, mine[p]);
           }
         }
       }
-      // keys of the group's nodes and their row weights, lane q of every
-      // 16-lane row holding row r0 + q NW of the node
-      int gkey[S][G];
+      // row weights of the group's nodes, lane q of every 16-lane row
+      // holding dimension-1 row r0 + q NW of the node
       float brr[S][G], bri[S][G];
       {
-        const int qoff = 4 * NW * (lane & 15);
+        const int qoff = 4 * (W + NW * (lane & 15));
 #pragma unroll
         for (int st = 0; st < S; st++)
 #pragma unroll
           for (int k = 0; k < G; k++) {
             const int p = k / NP, hk = k % NP;
-            gkey[st][k] = __builtin_amdgcn_readlane(ckey[st], base + k);
-            const int src = (gkey[st][k] >> 20) + 4 * hk * 32 + qoff;
+            const int src = ((gkey[st][k] >> 21) << 2) + 4 * hk * 32 + qoff;
             brr[st][k] = bperm_f(src, xr[st][p]);
             bri[st][k] = bperm_f(src, xi[st][p]);
           }
@@ -334,7 +370,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
       for (int k = 0; k < G; k++) {
 #pragma unroll
         for (int st = 0; st < S; st++) {
-          const int key = gkey[st][k];
+          const int key = gkey[st][k] & ~kDelta;
           float a0[SLOTS], a1[SLOTS];
 #pragma unroll
           for (int sl = 0; sl < SLOTS; sl++) {
@@ -373,8 +409,14 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
     else flush(st);
   }
   __syncthreads();
-  flush_tile<float, 3>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, sp.w, threadIdx.x,
-                       blockDim.x);
+  // tile cells in the unpadded numbering of the zone table: q -> q + (q /
+  // plane) (PP - plane); the quotient in FP32 is exact here ((q + 1/2) /
+  // plane stays at least 1 / (2 plane) from an integer)
+  const float inv_plane = 1.f / (float)plane;
+  const int pad = PP - plane;
+  flush_tile<float, 3>(
+      [&](int q) { return tile[q + (int)(((float)q + 0.5f) * inv_plane) * pad]; }, tiles, g, lo,
+      gg, tot, sp.w, threadIdx.x, blockDim.x);
 }
 
 }  // namespace nfft_hip

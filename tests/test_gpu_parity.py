@@ -180,14 +180,23 @@ def test_empty_and_edge_sizes():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant,d", [("runs2sbc", 3), ("runs2rs", 3), ("runs2d", 2)])
-def test_spreading_runs_edge_cases(variant, d, monkeypatch):
+@pytest.mark.parametrize("variant,d,bins", [("runs2sbc", 3, None), ("runs2rs", 3, None),
+                                            ("runs2sbc", 3, "8,8,7"), ("runs2rs", 3, "8,8,5"),
+                                            ("runs2sbc", 3, "4,8,8"), ("runs2sbc", 3, "8,4,6"),
+                                            ("runs2d", 2, None)])
+def test_spreading_runs_edge_cases(variant, d, bins, monkeypatch):
     """FP32 run-merging B^T kernels on work items that stress their node
     loops: 9001 nodes in one cell (runs across many 64-node chunks and
     several work items, streams of unequal length, partial last chunks), a
     lone node in a bin of its own (an item of one node: a stream without
-    nodes), nodes on the -0.5 edge, and uniform nodes in one corner."""
+    nodes), nodes on the -0.5 edge, and uniform nodes in one corner.  In 3-D
+    with the default 8^3 bins (runs of two cells along dimension 2) and with
+    other bin edges: odd in dimension 2 (runs of one cell; bins that do not
+    divide the grid, so tiles go to scratch without the zone order) and
+    even ones of other shapes (other plane pitches and lane tables)."""
     monkeypatch.setenv("NFFT_HIP_SPREAD", variant)
+    if bins:
+        monkeypatch.setenv("NFFT_HIP_BINS", bins)
     rng = np.random.default_rng(51)
     N = [16] * d
     x = np.concatenate([np.zeros((9001, d)), np.full((7, d), -0.5),
