@@ -273,7 +273,7 @@ __global__ __launch_bounds__(256) void k_fft_pass(const cplx<R> *gsrc, cplx<R> *
   gdst += (long)blockIdx.y * P.bs_g;
   if (fin) fin += (long)blockIdx.y * P.bs_c;
   if (fout) fout += (long)blockIdx.y * P.bs_c;
-  extern __shared__ unsigned char fft_smem[];
+  extern __shared__ __align__(16) unsigned char fft_smem[];
   cplx<R> *buf = reinterpret_cast<cplx<R> *>(fft_smem);
   const int ls = lds_line_stride(P.len);
   // the line length's twiddle table follows the lines in LDS when it fits

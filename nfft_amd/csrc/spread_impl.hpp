@@ -390,7 +390,7 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
   // right-hand side blockIdx.y of a batched launch
   g += (long)blockIdx.y * gg.bs_g;
   fout += (long)blockIdx.y * gg.bs_f;
-  extern __shared__ unsigned char b_smem[];
+  extern __shared__ __align__(16) unsigned char b_smem[];
   cplx<R> *tile = reinterpret_cast<cplx<R> *>(b_smem);
   const int4 sp = subs[blockIdx.x];
   int lo[D];
@@ -398,7 +398,10 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
   int tot = 1;
 #pragma unroll
   for (int t = 0; t < D; t++) tot *= gg.E[t];
-  // next 64-slot chunk of the work item (PAIR)
+  // next 64-slot chunk of the work item (PAIR).  (The dynamic tile is
+  // declared 16-byte aligned: after this 4-byte variable an unaligned
+  // declaration started it at a 4 (mod 8) offset, and every 8-byte LDS
+  // access of the 2-D kernel ran 4.6x slower.)
   __shared__ int next_chunk;
   if (threadIdx.x == 0) next_chunk = 0;
   if constexpr (D == 3) {
@@ -590,7 +593,7 @@ __global__ __launch_bounds__(256) void k_spread_tile(const cplx<R> *__restrict__
                                                      const R *__restrict__ frac,
                                                      const int4 *__restrict__ subs, BinGeom gg,
                                                      WinParams<R> wp, cplx<R> *__restrict__ tiles) {
-  extern __shared__ unsigned char b_smem[];
+  extern __shared__ __align__(16) unsigned char b_smem[];
   using Acc = LdsAcc<R>;
   typename Acc::elem *tile = reinterpret_cast<typename Acc::elem *>(b_smem);
   const int4 sp = subs[blockIdx.x];
@@ -688,7 +691,7 @@ __global__ __launch_bounds__(256) void k_bwave_tile(const cplx<R> *__restrict__ 
   constexpr int m = W / 2 - 1;
   constexpr int PAIRS = W * W;
   constexpr int SLOTS = (PAIRS + 63) / 64;
-  extern __shared__ unsigned char b_smem[];
+  extern __shared__ __align__(16) unsigned char b_smem[];
   cplx<R> *tile = reinterpret_cast<cplx<R> *>(b_smem);
   using Acc = LdsAcc<R>;
   typename Acc::elem *atile = reinterpret_cast<typename Acc::elem *>(b_smem);
@@ -818,7 +821,7 @@ __global__ __launch_bounds__(64) void k_spread_wave_priv(const cplx<R> *__restri
   constexpr int PAIRS = W * W;
   constexpr int SLOTS = (PAIRS + 63) / 64;
   constexpr int ROWS = D == 3 ? W : 1;  // first-dimension taps looped per lane
-  extern __shared__ unsigned char b_smem[];
+  extern __shared__ __align__(16) unsigned char b_smem[];
   cplx<R> *tile = reinterpret_cast<cplx<R> *>(b_smem);
   const int4 sp = subs[blockIdx.x];
   int lo[D];
@@ -943,7 +946,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_rows(const cplx<R> *__restri
   constexpr int PAIRS = W * W;
   constexpr int SLOTS = (PAIRS + 63) / 64;
   constexpr int RW = W / NW;  // rows per wavefront and node
-  extern __shared__ unsigned char b_smem[];
+  extern __shared__ __align__(16) unsigned char b_smem[];
   cplx<R> *tile = reinterpret_cast<cplx<R> *>(b_smem);
   const int4 sp = subs[blockIdx.x];
   int lo[3];
@@ -1087,7 +1090,7 @@ __global__ __launch_bounds__(256) void k_interp_rt(const cplx<R> *__restrict__ g
                                                    const R *__restrict__ frac,
                                                    const int4 *__restrict__ subs, BinGeom gg,
                                                    WinParams<R> wp, cplx<R> *__restrict__ tiles) {
-  extern __shared__ unsigned char b_smem[];
+  extern __shared__ __align__(16) unsigned char b_smem[];
   cplx<R> *tile = reinterpret_cast<cplx<R> *>(b_smem);
   const int W = 2 * gg.m + 2;
   const int4 sp = subs[blockIdx.x];
@@ -1196,7 +1199,7 @@ __global__ __launch_bounds__(256) void k_spread_rt(const cplx<R> *__restrict__ f
                                                    const R *__restrict__ frac,
                                                    const int4 *__restrict__ subs, BinGeom gg,
                                                    WinParams<R> wp, cplx<R> *__restrict__ tiles) {
-  extern __shared__ unsigned char b_smem[];
+  extern __shared__ __align__(16) unsigned char b_smem[];
   using Acc = LdsAcc<R>;
   typename Acc::elem *tile = reinterpret_cast<typename Acc::elem *>(b_smem);
   Acc acc;

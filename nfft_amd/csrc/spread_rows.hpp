@@ -38,7 +38,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_rows2(const cplx<R> *__restr
   constexpr int PAIRS = W * HW;      // (dim-1 tap, dim-2 pair) slots
   constexpr int SLOTS = (PAIRS + 63) / 64;
   constexpr int RW = W / NW;         // rows per wavefront and node
-  extern __shared__ unsigned char b_smem[];
+  extern __shared__ __align__(16) unsigned char b_smem[];
   cplx<R> *tile = reinterpret_cast<cplx<R> *>(b_smem);
   const int4 sp = subs[blockIdx.x];
   int lo[3];

@@ -106,7 +106,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
   fin += (long)blockIdx.y * gg.bs_f;
   g += (long)blockIdx.y * gg.bs_g;
   if (tiles) tiles += (long)blockIdx.y * gg.bs_t;
-  extern __shared__ unsigned char b_smem[];
+  extern __shared__ __align__(16) unsigned char b_smem[];
   float2 *tile = reinterpret_cast<float2 *>(b_smem);
   const int4 sp = subs[blockIdx.x];
   int lo[3];

@@ -55,7 +55,7 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
   fin += (long)blockIdx.y * gg.bs_f;
   g += (long)blockIdx.y * gg.bs_g;
   if (tiles) tiles += (long)blockIdx.y * gg.bs_t;
-  extern __shared__ unsigned char b_smem[];
+  extern __shared__ __align__(16) unsigned char b_smem[];
   C *tile = reinterpret_cast<C *>(b_smem);
   const int4 sp = subs[blockIdx.x];
   int lo[2];
