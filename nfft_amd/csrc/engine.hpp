@@ -82,6 +82,9 @@ template <typename R> struct WinParams {
   int poly_shared;
 };
 constexpr int kPolyDeg = 8;
+// widest window (taps) with interpolation slots of up to four nodes and
+// spreading runs of two cells; wider windows use pairs and one-cell runs
+constexpr int kSlotQuadMaxW = 10;
 constexpr int kPolyCoef = kPolyDeg + 1;
 
 // Twiddles and radix schedule of the FFT along one grid dimension.

@@ -519,7 +519,7 @@ void bin_nodes(Engine<R> &e) {
     hipLaunchKernelGGL(k_run_starts, dim3(gm), dim3(256), 0, s, b, run_id, M, run_start);
     // quads only where the kernel has their code (the shared symmetric
     // window table); otherwise pairs
-    const int Q = e.win.poly_shared && e.sym_window ? e.slot_nodes : 2;
+    const int Q = e.win.poly_shared && e.sym_window && 2 * gg.m + 2 <= kSlotQuadMaxW ? e.slot_nodes : 2;
     hipLaunchKernelGGL(k_slot_flags, dim3(gm), dim3(256), 0, s, run_id, run_start, M, Q, sflag);
     long_scan(sflag, sidx, M, sums + 101);
     int nslots = 0;

@@ -282,7 +282,7 @@ void Engine<R>::set_bins(const int *bs) {
   geom.sp_pp = geom.E[1] * geom.E[2];
   std::memset(geom.sp_lane, 255, sizeof(geom.sp_lane));
   if (d == 3 && 2 * m + 2 <= 20) {
-    geom.sp_gc = geom.bs[2] % 2 == 0 ? 2 : 1;
+    geom.sp_gc = geom.bs[2] % 2 == 0 && 2 * m + 2 <= kSlotQuadMaxW ? 2 : 1;
     spread_lane_table(2 * m + 2, geom.sp_gc, geom.E[0], geom.E[1], geom.E[2], geom.sp_pp,
                       geom.sp_lane);
   }

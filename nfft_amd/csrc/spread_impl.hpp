@@ -437,12 +437,18 @@ __global__ __launch_bounds__(256) void k_interp_tile(const cplx<R> *__restrict__
         for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
         if (wmax == 0) continue;
         if (wmax != 2) {
-          if (wmax == 4)
-            interp_slot_3d<W, 4>(tile, fout, perm, cell, frac, gg, wp, lo, PP, E2, pr, live);
-          else if (wmax == 3)
-            interp_slot_3d<W, 3>(tile, fout, perm, cell, frac, gg, wp, lo, PP, E2, pr, live);
-          else
+          // (slots of three or four nodes only for windows of up to 10 taps,
+          // where their registers fit; the plan builds pairs otherwise)
+          if constexpr (W <= kSlotQuadMaxW) {
+            if (wmax == 4)
+              interp_slot_3d<W, 4>(tile, fout, perm, cell, frac, gg, wp, lo, PP, E2, pr, live);
+            else if (wmax == 3)
+              interp_slot_3d<W, 3>(tile, fout, perm, cell, frac, gg, wp, lo, PP, E2, pr, live);
+            else
+              interp_slot_3d<W, 1>(tile, fout, perm, cell, frac, gg, wp, lo, PP, E2, pr, live);
+          } else {
             interp_slot_3d<W, 1>(tile, fout, perm, cell, frac, gg, wp, lo, PP, E2, pr, live);
+          }
           continue;
         }
       }
@@ -1634,9 +1640,12 @@ void bk_fill_one(BKernelTable<R> &k) {
   }
   // (the row weights of a wavefront's W/2 rows fit one 16-lane DPP row)
   if constexpr (std::is_same_v<R, float> && D == 3 && D * (2 * M_ + 2) <= 64 && M_ + 1 <= 16) {
-    k.spread_runs_2sbc[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false, 2>;
+    // (runs of two cells for windows of up to 10 taps; wider windows keep
+    // one, whose registers already fill the file)
+    constexpr int GC2 = 2 * M_ + 2 <= kSlotQuadMaxW ? 2 : 1;
+    k.spread_runs_2sbc[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false, GC2>;
     k.spread_runs_2sbc1[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false, 1>;
-    k.spread_runs_2rs[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 1, true, 2>;
+    k.spread_runs_2rs[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 1, true, GC2>;
   }
 }
 

@@ -209,6 +209,26 @@ def test_spreading_runs_edge_cases(variant, d, bins, monkeypatch):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("m,bins", [(5, None), (6, None), (7, None), (8, None), (5, "4,4,3"),
+                                    (3, "8,8,5")])
+def test_wide_and_odd_3d_fp32_windows(m, bins, monkeypatch):
+    """3-D FP32 B and B^T (run-merging spreading, slot interpolation) with
+    windows of 8 to 18 taps: above 10 taps one node per window pass (3 W >
+    32 lanes), two or three lane slots per wavefront from the plan's lane
+    table, interpolation pairs instead of quads and one-cell runs; with the
+    default bins and with odd dimension-2 bin edges, against the oracle."""
+    if bins:
+        monkeypatch.setenv("NFFT_HIP_BINS", bins)
+    N = [16, 16, 16]
+    x = W.kooshball(64, 48, seed=91)
+    fh = W.complex_normal(int(np.prod(N)), seed=92)
+    f = W.complex_normal(x.shape[0], seed=93)
+    (fg, fr, _), (hg, hr, _) = run_pair(N, x, fh, f, m, "single")
+    print("3-D FP32 m=%d bins %s: trafo rel l2 %.2e adjoint %.2e" % (m, bins, rel_errs(fg, fr)[0], rel_errs(hg, hr)[0]))
+    assert rel_errs(fg, fr)[0] < 1e-5 and rel_errs(hg, hr)[0] < 1e-5
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["double", "single"])
 def test_non_power_of_two_grid(precision):
     # oversampled lengths with factors 3, 5 exercise the mixed-radix FFT
