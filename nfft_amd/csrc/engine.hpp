@@ -19,7 +19,7 @@
 
 namespace nfft_hip {
 
-constexpr int kMaxDim = 4;
+constexpr int kMaxDim = 6;
 
 // Geometry shared by the node-binning and B kernels.
 struct BinGeom {
@@ -160,8 +160,9 @@ template <typename R> struct FftPass {
 template <typename R> struct Engine {
   // geometry
   int d = 0;
-  long N[kMaxDim] = {1, 1, 1, 1};
-  long n[kMaxDim] = {1, 1, 1, 1};
+  static_assert(kMaxDim == 6, "unit extents below");
+  long N[kMaxDim] = {1, 1, 1, 1, 1, 1};
+  long n[kMaxDim] = {1, 1, 1, 1, 1, 1};
   long N_total = 0, n_total = 0, M = 0;
   int m = 0;
   bool direct_only = false;  // some N_t <= m or n_t <= 2m+2: NDFT path

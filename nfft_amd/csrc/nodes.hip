@@ -189,7 +189,7 @@ __global__ void k_bin_weight(const int *__restrict__ count_of, BinGeom gg, long 
                              int *__restrict__ bucket_of, int *__restrict__ bucket_hist) {
   long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbins) return;
-  int bc[kMaxDim] = {0, 0, 0, 0};
+  int bc[kMaxDim] = {};
   long r = b;
   for (int t = gg.d - 1; t >= 0; t--) {
     bc[t] = (int)(r % gg.nb[t]);
@@ -201,7 +201,7 @@ __global__ void k_bin_weight(const int *__restrict__ count_of, BinGeom gg, long 
   for (int k = 0; k < zc; k++) {
     int rk = k;
     long nbl = 0;
-    int dig[kMaxDim] = {1, 1, 1, 1};
+    int dig[kMaxDim] = {};
     for (int t = gg.d - 1; t >= 0; t--) {
       dig[t] = rk % 3;
       rk /= 3;

@@ -321,7 +321,7 @@ void Engine<R>::set_bins(const int *bs) {
       std::vector<int> tab;
       std::vector<long> zbase(zc + 1, 0);
       for (int zz = 0; zz < zc; zz++) {
-        int dg[kMaxDim] = {0, 0, 0, 0}, r = zz;
+        int dg[kMaxDim] = {}, r = zz;
         for (int t = d - 1; t >= 0; t--) {
           dg[t] = r % 3;
           r /= 3;
@@ -345,7 +345,7 @@ void Engine<R>::set_bins(const int *bs) {
       // its zone 2 - dg_t per dimension, landing at core offset bs - m
       // (right neighbour's left part goes to the top of the core) or 0
       for (int k = 0; k < zc; k++) {
-        int dg[kMaxDim] = {1, 1, 1, 1}, r = k;
+        int dg[kMaxDim] = {}, r = k;
         for (int t = d - 1; t >= 0; t--) {
           dg[t] = r % 3;
           r /= 3;
@@ -445,7 +445,8 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
   // GPU (1-D 256 cells, 2-D n/32 clamped to [16, 64], 3-D 16), then shrink
   // the largest edge until the LDS tile fits the budget
   const size_t budget = 48 * 1024;
-  int bs[kMaxDim] = {1, 1, 1, 1};
+  int bs[kMaxDim];
+  for (int t = 0; t < kMaxDim; t++) bs[t] = 1;
   for (int t = 0; t < d; t++) {
     const long init = d == 1 ? 256 : d == 2 ? std::min<long>(64, std::max<long>(16, n[t] / 32)) : 16;
     bs[t] = (int)std::min<long>(n[t], init);
@@ -539,7 +540,7 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     ax.twiddle = twiddle_table(len);
     return ax;
   };
-  long split[kMaxDim] = {0, 0, 0, 0};
+  long split[kMaxDim] = {};
   std::vector<FftAxis<R>> sub1(d), sub2(d);
   std::vector<cplx<R> *> tw_full(d, nullptr);
   for (int t = 0; t < d; t++) {
@@ -1227,7 +1228,7 @@ struct PlanApi {
       family<R>::fatal("nfft: no HIP device available (this library runs on the GPU only)");
     if (p->flags & NFFT_OMP_BLOCKWISE_ADJOINT) p->flags |= NFFT_SORT_NODES;
     if (p->d < 1 || p->d > kMaxDim)
-      family<R>::fatal("nfft: dimension d = %td is not supported (1 <= d <= 4)", p->d);
+      family<R>::fatal("nfft: dimension d = %td is not supported (1 <= d <= 6)", p->d);
     p->N_total = 1;
     p->n_total = 1;
     for (NFFT_INT t = 0; t < p->d; t++) {

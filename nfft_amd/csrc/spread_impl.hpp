@@ -1658,6 +1658,14 @@ void bk_fill_dim(BKernelTable<R> &k, std::integer_sequence<int, Ms...>) {
   k.spread_rt[D - 1][1] = k_spread_rt<R, D, true>;
 }
 
+template <typename R, int D>
+void bk_fill_rt(BKernelTable<R> &t) {
+  t.interp_rt[D - 1][0] = k_interp_rt<R, D, false>;
+  t.interp_rt[D - 1][1] = k_interp_rt<R, D, true>;
+  t.spread_rt[D - 1][0] = k_spread_rt<R, D, false>;
+  t.spread_rt[D - 1][1] = k_spread_rt<R, D, true>;
+}
+
 template <typename R>
 BKernelTable<R> make_bkernel_table() {
   BKernelTable<R> t{};
@@ -1665,11 +1673,11 @@ BKernelTable<R> make_bkernel_table() {
   bk_fill_dim<R, 1>(t, seq{});
   bk_fill_dim<R, 2>(t, seq{});
   bk_fill_dim<R, 3>(t, seq{});
-  // d = 4: the generic runtime-m kernels only
-  t.interp_rt[3][0] = k_interp_rt<R, 4, false>;
-  t.interp_rt[3][1] = k_interp_rt<R, 4, true>;
-  t.spread_rt[3][0] = k_spread_rt<R, 4, false>;
-  t.spread_rt[3][1] = k_spread_rt<R, 4, true>;
+  // d = 4 .. 6: the generic runtime-m kernels only
+  bk_fill_rt<R, 4>(t);
+  bk_fill_rt<R, 5>(t);
+  bk_fill_rt<R, 6>(t);
+  static_assert(kMaxDim == 6, "runtime kernels for every d");
   return t;
 }
 

@@ -229,6 +229,27 @@ def test_wide_and_odd_3d_fp32_windows(m, bins, monkeypatch):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("d,N,n,m", [(5, [4] * 5, [8] * 5, 2), (6, [4] * 6, [8] * 6, 2),
+                                     (5, [4, 6, 4, 4, 4], [8, 12, 8, 8, 8], 2),
+                                     (5, [4] * 5, [8] * 5, 3)])
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_five_and_six_dimensions(d, N, n, m, precision):
+    """d = 5 and 6 (runtime-window B / B^T kernels with their tiles in HBM,
+    one band-pruned FFT pass per dimension, a length-12 axis among them)
+    against the oracle; at m = 3 with n_t = 2m + 2 the plan runs the direct
+    NDFT, as the reference does."""
+    M = 300
+    x = W.uniform(M, d, seed=70 + d)
+    fh = W.complex_normal(int(np.prod(N)), seed=80 + d)
+    f = W.complex_normal(M, seed=90 + d)
+    (fg, fr, _), (hg, hr, _) = run_pair(N, x, fh, f, m, precision, n=n)
+    tol = 1e-12 if precision == "double" else 1e-5
+    print("d=%d N=%s m=%d %s: trafo rel l2 %.2e adjoint %.2e"
+          % (d, N, m, precision, rel_errs(fg, fr)[0], rel_errs(hg, hr)[0]))
+    assert rel_errs(fg, fr)[0] < tol and rel_errs(hg, hr)[0] < tol
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["double", "single"])
 def test_non_power_of_two_grid(precision):
     # oversampled lengths with factors 3, 5 exercise the mixed-radix FFT
