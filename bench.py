@@ -276,7 +276,8 @@ def configs_leg(args):
     and the algorithmic HBM bytes of both directions over that time, and the
     relative l2 / l-infinity error of its results: FP32 against the FP64 plan
     on the same inputs (all nodes), FP64 against the FP64 oracle (all nodes
-    in the small 1-D case, 2000 sampled nodes otherwise)."""
+    in the small 1-D case, 2000 sampled nodes otherwise).  Plans use
+    FFTW_MEASURE (FFTW_ESTIMATE with --estimate)."""
     import torch
     import nfft_amd as nf
     import oracle as O
@@ -307,9 +308,14 @@ def configs_leg(args):
     O.set_threads(min(16, len(os.sched_getaffinity(0))))
     out = {}
     res64 = {}
+    # planned like the headline (FFTW_MEASURE unless --estimate): the tuner
+    # picks B / B^T kernels, bins and item sizes per workload (FP64 3-D
+    # interpolation, for one, is 2x faster on uniform nodes with one
+    # wavefront per node, 2.4x slower with it on the kooshball)
+    fftw_flags = (nf.FFTW_ESTIMATE if args.estimate else nf.FFTW_MEASURE) | nf.FFTW_DESTROY_INPUT
     for name, group, N, n, m, x, prec in cases:
         M = x.shape[0]
-        p = nf.Plan(N, M, n=n, m=m, flags=flags, precision=prec)
+        p = nf.Plan(N, M, n=n, m=m, flags=flags, precision=prec, fftw_flags=fftw_flags)
         p.x[:] = x
         p.set_stream(torch.cuda.current_stream().cuda_stream)
         p.precompute()
