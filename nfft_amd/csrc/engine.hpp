@@ -57,7 +57,7 @@ struct BinGeom {
   // 3-D run-merging spreading (spread_runs.hpp): the LDS tile's plane pitch
   // (cells; E[1] E[2] or padded) and the lane table, sp_lane[lane + 64 slot]
   // = the (dimension-0 tap, dimension-2 pair) index the lane owns (255:
-  // none); sp_gc = cells per run along dimension 2 (2 for an even bin edge)
+  // none); sp_gc = cells per run along dimension 2
   int sp_pp;
   int sp_gc;
   unsigned char sp_lane[192];
@@ -83,8 +83,9 @@ template <typename R> struct WinParams {
 };
 constexpr int kPolyDeg = 8;
 // widest window (taps) with interpolation slots of up to four nodes and
-// spreading runs of two cells; wider windows use pairs and one-cell runs
+// spreading runs of kRunCells cells; wider windows use pairs and one-cell runs
 constexpr int kSlotQuadMaxW = 10;
+constexpr int kRunCells = 3;
 constexpr int kPolyCoef = kPolyDeg + 1;
 
 // Twiddles and radix schedule of the FFT along one grid dimension.

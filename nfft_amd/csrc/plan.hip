@@ -277,12 +277,12 @@ void Engine<R>::set_bins(const int *bs) {
     }
   }
   // the run-merging spreading kernel's lane table (3-D, 2m + 2 taps; runs
-  // of two cells along dimension 2 for an even bin edge there)
+  // of kRunCells cells along dimension 2 where the bin edge allows)
   geom.sp_gc = 1;
   geom.sp_pp = geom.E[1] * geom.E[2];
   std::memset(geom.sp_lane, 255, sizeof(geom.sp_lane));
   if (d == 3 && 2 * m + 2 <= 20) {
-    geom.sp_gc = geom.bs[2] % 2 == 0 && 2 * m + 2 <= kSlotQuadMaxW ? 2 : 1;
+    geom.sp_gc = geom.bs[2] >= kRunCells && 2 * m + 2 <= kSlotQuadMaxW ? kRunCells : 1;
     spread_lane_table(2 * m + 2, geom.sp_gc, geom.E[0], geom.E[1], geom.E[2], geom.sp_pp,
                       geom.sp_lane);
   }

@@ -1608,7 +1608,8 @@ template <typename R> struct BKernelTable {
   // d = 3, float only, 128 threads: run merging, two wavefronts per tile,
   // row weights by DPP row broadcast; two node streams per wavefront (2sbc)
   // or one stream whose runs load and store their tile cells (2rs); runs of
-  // two cells along dimension 2 (an even bin edge there), else of one (2sbc1)
+  // kRunCells cells along dimension 2 (a bin edge of at least that many
+  // there), else of one (2sbc1)
   fn spread_runs_2sbc[kMaxStaticM + 1];
   fn spread_runs_2sbc1[kMaxStaticM + 1];
   fn spread_runs_2rs[kMaxStaticM + 1];
@@ -1640,12 +1641,12 @@ void bk_fill_one(BKernelTable<R> &k) {
   }
   // (the row weights of a wavefront's W/2 rows fit one 16-lane DPP row)
   if constexpr (std::is_same_v<R, float> && D == 3 && D * (2 * M_ + 2) <= 64 && M_ + 1 <= 16) {
-    // (runs of two cells for windows of up to 10 taps; wider windows keep
-    // one, whose registers already fill the file)
-    constexpr int GC2 = 2 * M_ + 2 <= kSlotQuadMaxW ? 2 : 1;
-    k.spread_runs_2sbc[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false, GC2>;
+    // (runs of kRunCells cells for windows of up to 10 taps; wider windows
+    // keep one, whose registers already fill the file)
+    constexpr int GC = 2 * M_ + 2 <= kSlotQuadMaxW ? kRunCells : 1;
+    k.spread_runs_2sbc[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false, GC>;
     k.spread_runs_2sbc1[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 2, false, 1>;
-    k.spread_runs_2rs[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 1, true, GC2>;
+    k.spread_runs_2rs[M_] = k_spread_runs<2 * M_ + 2, 2, 8, 1, true, GC>;
   }
 }
 
@@ -1710,11 +1711,12 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
       return k.spread_runs2d[gg.m];
     }
     if (spread && (variant == 10 || variant == 13) && gg.d == 3 && k.spread_runs_2sbc[gg.m]) {
-      // the tile with its padded plane pitch; runs of one cell for an odd
-      // dimension-2 bin edge (the load-store form exists for two only)
+      // the tile with its padded plane pitch; runs of one cell for a
+      // dimension-2 bin edge below kRunCells (the load-store form exists for
+      // kRunCells only)
       threads = 128;
       lds = (size_t)gg.E[0] * gg.sp_pp * sizeof(cplx<R>);
-      if (gg.sp_gc != 2) return k.spread_runs_2sbc1[gg.m];
+      if (gg.sp_gc == 1) return k.spread_runs_2sbc1[gg.m];
       return variant == 13 ? k.spread_runs_2rs[gg.m] : k.spread_runs_2sbc[gg.m];
     }
     if (spread && variant >= 3 && gg.d == 3 && k.spread_rows[gg.m]) {

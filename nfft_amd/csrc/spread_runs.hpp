@@ -71,11 +71,13 @@ __device__ __forceinline__ void rows_bcast(float2 (&acc)[RW][SLOTS][2], float br
 // and the other wavefronts own other rows, so the load-store pair is
 // race-free.
 // GC: a run is the nodes of GC adjacent cells along dimension 2 (same
-// dimension-0 and dimension-1 cell, dimension-2 cells 2k .. 2k+GC-1 of the
-// bin), whose windows fit W + GC - 1 dimension-2 taps; a node at dimension-2
-// offset delta in its group takes its taps from lane tap - delta.  With
-// GC = 2 a run holds the nodes of two cells, so runs are about twice as long
-// and flush half as often (needs an even bin edge in dimension 2).
+// dimension-0 and dimension-1 cell, dimension-2 cells b .. b+GC-1 of the bin
+// with b = min(GC floor(c / GC), bs - GC), so that the group's taps stay in
+// the tile row), whose windows fit W + GC - 1 dimension-2 taps; a node at
+// dimension-2 offset delta in its group takes its taps from lane tap - delta.
+// With GC = 3 a run holds the nodes of three cells, so runs are about three
+// times as long and flush a third as often, and for even W the (W + 2) / 2
+// pairs cover the W + 2 taps exactly (needs a bin edge of 3 or more).
 template <int W, int NW, int G, int S, bool RS = false, int GC = 1>
 __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restrict__ fin,
                                                          float2 *__restrict__ g,
@@ -86,7 +88,9 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
                                                          BinGeom gg, WinParams<float> wp,
                                                          float2 *__restrict__ tiles) {
   static_assert(W % 2 == 0 && W % NW == 0 && 3 * W <= 64, "W = 2m+2 taps, rows split evenly");
-  static_assert(GC == 1 || GC == 2, "runs of one or two cells");
+  static_assert(GC >= 1 && GC <= 3, "runs of one to three cells");
+  // the dimension-2 pairs cover the run's taps exactly (no cell past them)
+  constexpr bool FULL = (W + GC - 1) % 2 == 0;
   constexpr int HW = (W + GC) / 2;  // dimension-2 pairs covering W + GC - 1 taps
   constexpr int PAIRS = W * HW;
   constexpr int SLOTS = (PAIRS + 63) / 64;
@@ -127,27 +131,28 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
   // lane table; c1[sl]: the pair's second cell lies inside the run's taps
   int off[SLOTS];
   bool pv[SLOTS], c1[SLOTS];
-  // (adr_v*s: the node one cell further along dimension 2 in its group,
-  // taps shifted by one; separate arrays, so that choosing between them
+  // (adr_t[k]: the lane holding dimension-2 tap pb + 1 - k, so that a node at
+  // offset delta in its group reads its pair's taps from adr_t[delta + 1]
+  // and adr_t[delta]; separate named arrays, so that choosing between them
   // stays a select and never becomes a dynamically indexed private array)
-  int adr_u[NP][SLOTS], adr_v0[NP][SLOTS], adr_v1[NP][SLOTS], adr_v0s[NP][SLOTS],
-      adr_v1s[NP][SLOTS];
+  int adr_u[NP][SLOTS], adr_p1[NP][SLOTS], adr_0[NP][SLOTS], adr_m1[NP][SLOTS],
+      adr_m2[NP][SLOTS];
 #pragma unroll
   for (int sl = 0; sl < SLOTS; sl++) {
     const int q = gg.sp_lane[lane + 64 * sl];
     pv[sl] = q < PAIRS;
     const int pa = pv[sl] ? q / HW : 0;
     const int pb = pv[sl] ? 2 * (q - (q / HW) * HW) : 0;
-    c1[sl] = pv[sl] && pb + 1 < W + GC - 1;
+    c1[sl] = pv[sl] && (FULL || pb + 1 < W + GC - 1);
     off[sl] = pa * PP + pb;
 #pragma unroll
     for (int h = 0; h < NP; h++) {
       adr_u[h][sl] = 4 * (h * 32 + pa);
       auto tap_adr = [&](int t) { return 4 * (h * 32 + (t >= 0 && t < W ? 2 * W + t : ZL)); };
-      adr_v0[h][sl] = tap_adr(pb);
-      adr_v1[h][sl] = tap_adr(pb + 1);
-      adr_v0s[h][sl] = tap_adr(pb - 1);
-      adr_v1s[h][sl] = tap_adr(pb);
+      adr_p1[h][sl] = tap_adr(pb + 1);
+      adr_0[h][sl] = tap_adr(pb);
+      adr_m1[h][sl] = tap_adr(pb - 1);
+      adr_m2[h][sl] = tap_adr(pb - 2);
     }
   }
   // window tap evaluated by this lane in a pass: node half h, dimension wt,
@@ -188,10 +193,10 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
   // compare without delta), from the stream's first node on (a stream
   // without nodes holds the item's first node's corner and only ever adds
   // zeros there)
-  constexpr int kDelta = 1 << 20;
+  constexpr int kDelta = 3 << 20;
   int cur[S];
   auto cell_at = [&](int st, int sl, int j) {
-    return tile + (cur[st] & 0xfffff) + ((cur[st] >> 21) + j * NW) * E2 + off[sl];
+    return tile + (cur[st] & 0xfffff) + ((cur[st] >> 22) + j * NW) * E2 + off[sl];
   };
   // the run's sums into the tile (read-modify-write), accumulators cleared
   auto flush = [&](int st) {
@@ -204,13 +209,13 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
         for (int j = 0; j < RW; j++) {
           const float2 *t = cell_at(st, sl, j);
           v[j][0] = t[0];
-          v[j][1] = c1[sl] ? t[1] : float2{0.f, 0.f};
+          v[j][1] = FULL || c1[sl] ? t[1] : float2{0.f, 0.f};
         }
 #pragma unroll
         for (int j = 0; j < RW; j++) {
           float2 *t = cell_at(st, sl, j);
           t[0] = float2{v[j][0].x + acc[st][j][sl][0].x, v[j][0].y + acc[st][j][sl][0].y};
-          if (c1[sl])
+          if (FULL || c1[sl])
             t[1] = float2{v[j][1].x + acc[st][j][sl][1].x, v[j][1].y + acc[st][j][sl][1].y};
         }
       }
@@ -228,7 +233,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
         for (int j = 0; j < RW; j++) {
           float2 *t = cell_at(st, sl, j);
           t[0] = acc[st][j][sl][0];
-          if (c1[sl]) t[1] = acc[st][j][sl][1];
+          if (FULL || c1[sl]) t[1] = acc[st][j][sl][1];
         }
       }
   };
@@ -240,7 +245,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
         for (int j = 0; j < RW; j++) {
           const float2 *t = cell_at(st, sl, j);
           acc[st][j][sl][0] = t[0];
-          acc[st][j][sl][1] = c1[sl] ? t[1] : float2{0.f, 0.f};
+          acc[st][j][sl][1] = FULL || c1[sl] ? t[1] : float2{0.f, 0.f};
         }
       }
   };
@@ -284,8 +289,9 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
     // the dimension-1 cell determines r0, so keys of one wavefront compare
     // as corners
     const int r0 = ((wv - a[1]) % NW + NW) % NW;
-    const int dl = GC == 2 ? (a[2] & 1) : 0;
-    ckey[st] = (a[0] * PP + a[1] * E2 + a[2] - dl) | (dl * kDelta) | (r0 << 21);
+    const int g2 = GC == 1 ? a[2] : min(a[2] - a[2] % GC, gg.bs[2] - GC);
+    const int dl = a[2] - g2;
+    ckey[st] = (a[0] * PP + a[1] * E2 + g2) | (dl << 20) | (r0 << 22);
     cf[st] = first + lane < sn[st] ? nf[st] : float2{0.f, 0.f};
   };
   int longest = 0;
@@ -340,12 +346,12 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
 #pragma unroll
         for (int k = 0; k < G; k++) {
           const int p = k / NP, hk = k % NP;
-          const bool dl = GC == 2 && (gkey[st][k] & kDelta);
+          const int dl = GC == 1 ? 0 : (gkey[st][k] >> 20) & 3;
 #pragma unroll
           for (int sl = 0; sl < SLOTS; sl++) {
             const float u = bperm_f(adr_u[hk][sl], mine[p]);
-            const int a0 = dl ? adr_v0s[hk][sl] : adr_v0[hk][sl];
-            const int a1 = dl ? adr_v1s[hk][sl] : adr_v1[hk][sl];
+            const int a0 = dl == 0 ? adr_0[hk][sl] : (dl == 1 ? adr_m1[hk][sl] : adr_m2[hk][sl]);
+            const int a1 = dl == 0 ? adr_p1[hk][sl] : (dl == 1 ? adr_0[hk][sl] : adr_m1[hk][sl]);
             w0[st][k][sl] = u * bperm_f(a0, mine[p]);
             w1[st][k][sl] = u * bperm_f(a1, mine[p]);
           }
@@ -361,7 +367,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
 #pragma unroll
           for (int k = 0; k < G; k++) {
             const int p = k / NP, hk = k % NP;
-            const int src = ((gkey[st][k] >> 21) << 2) + 4 * hk * 32 + qoff;
+            const int src = ((gkey[st][k] >> 22) << 2) + 4 * hk * 32 + qoff;
             brr[st][k] = bperm_f(src, xr[st][p]);
             bri[st][k] = bperm_f(src, xi[st][p]);
           }

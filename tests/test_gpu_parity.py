@@ -183,17 +183,19 @@ def test_empty_and_edge_sizes():
 @pytest.mark.parametrize("variant,d,bins", [("runs2sbc", 3, None), ("runs2rs", 3, None),
                                             ("runs2sbc", 3, "8,8,7"), ("runs2rs", 3, "8,8,5"),
                                             ("runs2sbc", 3, "4,8,8"), ("runs2sbc", 3, "8,4,6"),
-                                            ("runs2d", 2, None)])
+                                            ("runs2sbc", 3, "8,8,3"), ("runs2sbc", 3, "8,8,2"),
+                                            ("runs2rs", 3, "8,8,1"), ("runs2d", 2, None)])
 def test_spreading_runs_edge_cases(variant, d, bins, monkeypatch):
     """FP32 run-merging B^T kernels on work items that stress their node
     loops: 9001 nodes in one cell (runs across many 64-node chunks and
     several work items, streams of unequal length, partial last chunks), a
     lone node in a bin of its own (an item of one node: a stream without
     nodes), nodes on the -0.5 edge, and uniform nodes in one corner.  In 3-D
-    with the default 8^3 bins (runs of two cells along dimension 2) and with
-    other bin edges: odd in dimension 2 (runs of one cell; bins that do not
-    divide the grid, so tiles go to scratch without the zone order) and
-    even ones of other shapes (other plane pitches and lane tables)."""
+    with the default 8^3 bins (runs of three cells along dimension 2, the
+    last group of a bin row clamped to cells 5..7) and with other bin edges:
+    7, 5 and 3 in dimension 2 (other clamped groups; bins that do not divide
+    the grid, so tiles go to scratch without the zone order), 2 and 1 (runs
+    of one cell), and other shapes (other plane pitches and lane tables)."""
     monkeypatch.setenv("NFFT_HIP_SPREAD", variant)
     if bins:
         monkeypatch.setenv("NFFT_HIP_BINS", bins)
