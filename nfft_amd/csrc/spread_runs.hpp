@@ -383,7 +383,9 @@ __global__ __launch_bounds__(64 * NW) void k_spread_runs(const float2 *__restric
             a0[sl] = w0[st][k][sl];
             a1[sl] = w1[st][k][sl];
           }
-          if (key != cur[st]) {
+          // most nodes continue their run: the flush is placed off the
+          // straight-line path, so a node without one takes no branch
+          if (__builtin_expect(key != cur[st], 0)) {
             if constexpr (RS) {
               store(st);
               cur[st] = key;

@@ -230,7 +230,7 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
 #pragma unroll
         for (int st = 0; st < S; st++) {
           const int key = __builtin_amdgcn_readlane(ckey[st], base + k);
-          if (key != cur[st]) {
+          if (__builtin_expect(key != cur[st], 0)) {
             flush(st);
             cur[st] = key;
           }
