@@ -66,15 +66,14 @@ struct BinGeom {
 // Kaiser-Bessel window of one dimension, scaled by 1/phi(0) so that the B
 // weights stay O(1) (the scale is folded into the deconvolution factors).
 template <typename R> struct WinParams {
-  R b[kMaxDim];        // shape parameter pi (2 - 1/sigma_t)
-  R inv_den[kMaxDim];  // 1 / (1 - exp(-2 b m))
-  R tail[kMaxDim];     // 2 exp(-b m) / (1 - exp(-2 b m))
   // Per dimension t and tap w, the scaled window psi(frac + m - w) as a
-  // degree kPolyDeg polynomial in (frac - 1/2), coefficients lowest first at
-  // poly[(t * (2m+2) + w) * kPolyCoef + k]; fitted on the host in long double
-  // (Chebyshev interpolation; at degree 8 the largest error is 1.7e-8 of the
-  // peak at m = 4, sigma = 2, 1.9e-7 at m = 2 and below 5e-8 for m >= 3,
-  // against FP32's 6e-8 rounding of values near the peak).
+  // polynomial in (frac - 1/2) of degree poly_deg_v<R>, coefficients lowest
+  // first at poly[(t * (2m+2) + w) * (poly_deg_v<R> + 1) + k]; fitted on the
+  // host in long double (Chebyshev interpolation).  FP32, degree 8: the
+  // largest error is 1.7e-8 of the peak at m = 4, sigma = 2, 1.9e-7 at m = 2
+  // and below 5e-8 for m >= 3, against FP32's 6e-8 rounding of values near
+  // the peak.  FP64, degree 20: at most 1.9e-16 of the peak for m = 1..16 and
+  // sigma = 1.25, 2, 4 with the FP64 Horner evaluation, FP64 rounding.
   // The table is mirror-symmetric: tap 2m+1-w at t is tap w at -t, so its
   // coefficients are those of tap w with odd powers negated.
   const R *poly;
@@ -82,6 +81,8 @@ template <typename R> struct WinParams {
   int poly_shared;
 };
 constexpr int kPolyDeg = 8;
+constexpr int kPolyDeg64 = 20;
+template <typename R> constexpr int poly_deg_v = sizeof(R) == 4 ? kPolyDeg : kPolyDeg64;
 // widest window (taps) with interpolation slots of up to four nodes and
 // spreading runs of kRunCells cells; wider windows use pairs and one-cell runs
 constexpr int kSlotQuadMaxW = 10;

@@ -96,12 +96,14 @@ static long double psi_scaled(long double y, int m, long double b) {
 }
 
 // Per tap w = 0..2m+1 of one dimension, psi(frac + m - w) for frac in [0,1)
-// as a polynomial in t = frac - 1/2 of degree kPolyDeg: Chebyshev
-// interpolation at 32 points, converted to monomials in t.  The window is an
-// entire function of y^2, so the fit converges geometrically.
+// as a polynomial in t = frac - 1/2 of degree poly_deg_v<R> (8 for FP32, 20
+// for FP64): Chebyshev interpolation at 32 (FP32) or 48 (FP64) points,
+// converted to monomials in t.  The window is an entire function of y^2, so
+// the fit converges geometrically.
 template <typename R>
 static void fit_window_polys(int m, long double b, std::vector<R> &out) {
-  const int K = 32, DEG = kPolyDeg;
+  constexpr int DEG = poly_deg_v<R>;
+  constexpr int K = DEG <= 8 ? 32 : 48;
   const size_t first = out.size();
   for (int w = 0; w < 2 * m + 2; w++) {
     long double f[K], u[K];
@@ -418,9 +420,6 @@ void Engine<R>::create(int d_, const long *N_, const long *n_, long M_, int m_) 
     long double sigma = (long double)n[t] / (long double)N[t];
     long double b = kPiL * (2.0L - 1.0L / sigma);
     long double peak = sinhl(b * m) / (kPiL * m);  // phi(0)
-    win.b[t] = (R)b;
-    win.inv_den[t] = (R)(1.0L / (-expm1l(-2.0L * b * m)));
-    win.tail[t] = (R)(2.0L * expl(-b * m) / (-expm1l(-2.0L * b * m)));
     for (long ks = 0; ks < N[t]; ks++) pinv.push_back((R)(peak / phi_hut(n[t], ks - N[t] / 2, m, b)));
   }
   NFFT_HIP_CHECK(R, hipMalloc(&phi_inv, sizeof(R) * pinv.size()));

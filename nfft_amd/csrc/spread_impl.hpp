@@ -173,28 +173,15 @@ __device__ __forceinline__ void node_window(const int *__restrict__ cell,
                                             const R *__restrict__ frac, long p, long M,
                                             const WinParams<R> &wp, int (&c)[D],
                                             R (&psi)[D][W]) {
-  constexpr int m = W / 2 - 1;
 #pragma unroll
   for (int t = 0; t < D; t++) {
     c[t] = cell[t * M + p];
     const R fr = frac[t * M + p];
-    if constexpr (sizeof(R) == 4) {
-      // per-tap polynomials in (frac - 1/2); coefficients are uniform over
-      // the wavefront and come from scalar loads
-      const R tt = fr - R(0.5);
+    // per-tap polynomials in (frac - 1/2); coefficients are uniform over the
+    // wavefront and come from scalar loads
+    constexpr int PD = poly_deg_v<R>;
 #pragma unroll
-      for (int w = 0; w < W; w++) {
-        const R *pc = wp.poly + (t * W + w) * kPolyCoef;
-        R v = pc[kPolyDeg];
-#pragma unroll
-        for (int k = kPolyDeg - 1; k >= 0; k--) v = fmaf(v, tt, pc[k]);
-        psi[t][w] = v;
-      }
-    } else {
-#pragma unroll
-      for (int w = 0; w < W; w++)
-        psi[t][w] = kb_psi<R>(fr + R(m - w), R(m), wp.b[t], wp.inv_den[t]);
-    }
+    for (int w = 0; w < W; w++) psi[t][w] = tap_poly<R, PD>(wp.poly + (t * W + w) * (PD + 1), fr);
   }
 }
 
@@ -694,7 +681,6 @@ __global__ __launch_bounds__(256) void k_bwave_tile(const cplx<R> *__restrict__ 
                                                     const int4 *__restrict__ subs, BinGeom gg,
                                                     WinParams<R> wp, cplx<R> *__restrict__ tiles) {
   static_assert(D == 2 || D == 3, "wave-per-node kernels cover d = 2, 3");
-  constexpr int m = W / 2 - 1;
   constexpr int PAIRS = W * W;
   constexpr int SLOTS = (PAIRS + 63) / 64;
   extern __shared__ __align__(16) unsigned char b_smem[];
@@ -731,13 +717,16 @@ __global__ __launch_bounds__(256) void k_bwave_tile(const cplx<R> *__restrict__ 
     pb[s] = pv[s] ? q - (q / W) * W : 0;
     off[s] = D == 3 ? pa[s] * E2 + pb[s] : pa[s] * E1 + pb[s];
   }
-  // window value owned by this lane: dimension lane / W, tap lane % W
+  // window value owned by this lane: dimension lane / W, tap lane % W, its
+  // polynomial in registers
   const int wt = lane / W < D ? lane / W : D - 1;
   const int ww = lane - (lane / W) * W;
+  TapCoef<R, poly_deg_v<R>> tc;
+  tc.load(wp.poly + (wt * W + ww) * (poly_deg_v<R> + 1));
   for (int i = wv; i < sp.z; i += nw) {
     const long p = (long)sp.y + i;
     R mine = R(0);
-    if (lane < D * W) mine = kb_psi<R>(frac[wt * gg.M + p] + R(m - ww), R(m), wp.b[wt], wp.inv_den[wt]);
+    if (lane < D * W) mine = tc(frac[wt * gg.M + p]);
     int a[D];
 #pragma unroll
     for (int t = 0; t < D; t++) a[t] = cell[t * gg.M + p] - lo[t];
@@ -823,7 +812,6 @@ __global__ __launch_bounds__(64) void k_spread_wave_priv(const cplx<R> *__restri
                                                          BinGeom gg, WinParams<R> wp,
                                                          cplx<R> *__restrict__ tiles) {
   static_assert(D == 2 || D == 3, "private-tile kernels cover d = 2, 3");
-  constexpr int m = W / 2 - 1;
   constexpr int PAIRS = W * W;
   constexpr int SLOTS = (PAIRS + 63) / 64;
   constexpr int ROWS = D == 3 ? W : 1;  // first-dimension taps looped per lane
@@ -852,6 +840,8 @@ __global__ __launch_bounds__(64) void k_spread_wave_priv(const cplx<R> *__restri
   }
   const int wt = lane / W < D ? lane / W : D - 1;
   const int ww = lane - (lane / W) * W;
+  TapCoef<R, poly_deg_v<R>> tc;
+  tc.load(wp.poly + (wt * W + ww) * (poly_deg_v<R> + 1));
   const int rowstride = D == 3 ? E1 * E2 : 0;
   // node data arrive 64 nodes at a time, one node per lane (coalesced), the
   // next chunk loading while the current one is processed; node k of a
@@ -885,7 +875,7 @@ __global__ __launch_bounds__(64) void k_spread_wave_priv(const cplx<R> *__restri
     const R frw = D == 3 ? (wt == 0 ? frk[0] : wt == 1 ? frk[1] : frk[D - 1])
                          : (wt == 0 ? frk[0] : frk[D - 1]);
     R mine = R(0);
-    if (lane < D * W) mine = kb_psi<R>(frw + R(m - ww), R(m), wp.b[wt], wp.inv_den[wt]);
+    if (lane < D * W) mine = tc(frw);
     R wpair[SLOTS];
 #pragma unroll
     for (int s = 0; s < SLOTS; s++) {
@@ -948,7 +938,6 @@ __global__ __launch_bounds__(64 * NW) void k_spread_rows(const cplx<R> *__restri
                                                          BinGeom gg, WinParams<R> wp,
                                                          cplx<R> *__restrict__ tiles) {
   static_assert(D == 3 && W % NW == 0, "row-partitioned spreading covers d = 3");
-  constexpr int m = W / 2 - 1;
   constexpr int PAIRS = W * W;
   constexpr int SLOTS = (PAIRS + 63) / 64;
   constexpr int RW = W / NW;  // rows per wavefront and node
@@ -978,6 +967,8 @@ __global__ __launch_bounds__(64 * NW) void k_spread_rows(const cplx<R> *__restri
   }
   const int wt = lane / W < 3 ? lane / W : 2;
   const int ww = lane - (lane / W) * W;
+  TapCoef<R, poly_deg_v<R>> tc;
+  tc.load(wp.poly + (wt * W + ww) * (poly_deg_v<R> + 1));
   int cc[3], nc[3];
   R cfr[3], nfr[3];
   cplx<R> cf, nf;
@@ -1004,12 +995,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_rows(const cplx<R> *__restri
         a[t] = __builtin_amdgcn_readlane(cc[t], k) - lo[t];
       }
       const R frw = wt == 0 ? frk[0] : (wt == 1 ? frk[1] : frk[2]);
-      R mine = R(0);
-      if constexpr (sizeof(R) == 4) {
-        mine = kb_psi_fast(frw + R(m - ww), R(m), wp.b[wt], wp.inv_den[wt], wp.tail[wt]);
-      } else {
-        mine = kb_psi<R>(frw + R(m - ww), R(m), wp.b[wt], wp.inv_den[wt]);
-      }
+      const R mine = tc(frw);
       R wpair[SLOTS];
       int addr[SLOTS];
       const int row_base = (a[1] * E2 + a[2]);
@@ -1079,11 +1065,11 @@ __device__ __forceinline__ void node_window_rt(const int *__restrict__ cell,
                                                const BinGeom &gg, const WinParams<R> &wp,
                                                int (&c)[D], R (&psi)[D][kMaxW]) {
   const int W = 2 * gg.m + 2;
+  constexpr int PD = poly_deg_v<R>;
   for (int t = 0; t < D; t++) {
     c[t] = cell[t * gg.M + p];
     const R fr = frac[t * gg.M + p];
-    for (int w = 0; w < W; w++)
-      psi[t][w] = kb_psi<R>(fr + R(gg.m - w), R(gg.m), wp.b[t], wp.inv_den[t]);
+    for (int w = 0; w < W; w++) psi[t][w] = tap_poly<R, PD>(wp.poly + (t * W + w) * (PD + 1), fr);
   }
 }
 

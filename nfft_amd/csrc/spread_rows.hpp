@@ -33,7 +33,6 @@ __global__ __launch_bounds__(64 * NW) void k_spread_rows2(const cplx<R> *__restr
                                                           BinGeom gg, WinParams<R> wp,
                                                           cplx<R> *__restrict__ tiles) {
   static_assert(W % 2 == 0 && W % NW == 0, "W = 2m+2 rows split evenly");
-  constexpr int m = W / 2 - 1;
   constexpr int HW = W / 2;          // tap pairs along dimension 2
   constexpr int PAIRS = W * HW;      // (dim-1 tap, dim-2 pair) slots
   constexpr int SLOTS = (PAIRS + 63) / 64;
@@ -64,16 +63,9 @@ __global__ __launch_bounds__(64 * NW) void k_spread_rows2(const cplx<R> *__restr
   // window value owned by this lane: dimension wt, tap ww
   const int wt = lane / W < 3 ? lane / W : 2;
   const int ww = lane - (lane / W) * W;
-  const R wb = wt == 0 ? wp.b[0] : (wt == 1 ? wp.b[1] : wp.b[2]);
-  const R wden = wt == 0 ? wp.inv_den[0] : (wt == 1 ? wp.inv_den[1] : wp.inv_den[2]);
-  const R wy = R(m - ww);
-  // FP32: this lane's window polynomial (tap ww of dimension wt) in registers
-  R coef[kPolyCoef];
-  if constexpr (sizeof(R) == 4) {
-    const R *pc = wp.poly + (wt * W + ww) * kPolyCoef;
-#pragma unroll
-    for (int k = 0; k < kPolyCoef; k++) coef[k] = pc[k];
-  }
+  // this lane's window polynomial (tap ww of dimension wt) in registers
+  TapCoef<R, poly_deg_v<R>> tc;
+  tc.load(wp.poly + (wt * W + ww) * (poly_deg_v<R> + 1));
 
   int cc[3], nc[3];
   R cfr[3], nfr[3];
@@ -100,15 +92,7 @@ __global__ __launch_bounds__(64 * NW) void k_spread_rows2(const cplx<R> *__restr
       const int a1 = __builtin_amdgcn_readlane(cc[1], k) - lo[1];
       const int a2 = __builtin_amdgcn_readlane(cc[2], k) - lo[2];
       const R frw = wt == 0 ? fr0 : (wt == 1 ? fr1 : fr2);
-      R mine;
-      if constexpr (sizeof(R) == 4) {
-        const R tt = frw - R(0.5);
-        mine = coef[kPolyDeg];
-#pragma unroll
-        for (int k = kPolyDeg - 1; k >= 0; k--) mine = fmaf(mine, tt, coef[k]);
-      } else {
-        mine = kb_psi<R>(frw + wy, R(m), wb, wden);
-      }
+      const R mine = tc(frw);
       // pair weights psi1[pa] * psi2[pb], psi1[pa] * psi2[pb+1]
       R w0[SLOTS], w1[SLOTS];
       int addr[SLOTS];

@@ -11,8 +11,8 @@
 // in program order, so the streams' updates of shared cells do not race.
 //
 // Window values: G nodes at a time, each lane evaluates one tap (dimension
-// wt, tap ww) of one node, two nodes per pass when 2 W <= 32: FP32 as a
-// per-tap polynomial, FP64 in closed form.  Lanes of dimension 0 fold the
+// wt, tap ww) of one node, two nodes per pass when 2 W <= 32, as a per-tap
+// polynomial in (frac - 1/2).  Lanes of dimension 0 fold the
 // node value in (psi0 f_j, complex), and every lane fetches its factors by
 // ds_bpermute (two per FP64 value).
 #pragma once
@@ -88,15 +88,9 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
   const int li = NP == 2 ? lane & 31 : lane;
   const int wt = li / W < 2 ? li / W : 1;
   const int ww = li - (li / W) * W;
-  float coef[kPolyCoef];
-  if constexpr (sizeof(R) == 4) {
-    const R *pc = wp.poly + (wt * W + ww) * kPolyCoef;
-#pragma unroll
-    for (int k = 0; k < kPolyCoef; k++) coef[k] = pc[k];
-  }
-  // FP64: psi(frac + m - ww) of dimension wt in closed form
-  const R wb = wp.b[wt], wden = wp.inv_den[wt];
-  const R wm = R(W / 2 - 1);
+  // its tap polynomial in registers
+  TapCoef<R, poly_deg_v<R>> tc;
+  tc.load(wp.poly + (wt * W + ww) * (poly_deg_v<R> + 1));
 
   int sb[S], sn[S];
   {
@@ -162,8 +156,8 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
   auto convert = [&](int st, int first) {
     ckey[st] = (nc[st][0] - lo[0]) * E1 + (nc[st][1] - lo[1]);
 #pragma unroll
-    // FP32: frac - 1/2 (the polynomials' variable); FP64: frac
-    for (int t = 0; t < 2; t++) cfr[st][t] = sizeof(R) == 4 ? nfr[st][t] - R(0.5) : nfr[st][t];
+    // frac - 1/2, the polynomials' variable
+    for (int t = 0; t < 2; t++) cfr[st][t] = nfr[st][t] - R(0.5);
     cf[st] = first + lane < sn[st] ? nf[st] : C{R(0), R(0)};
   };
   int longest = 0;
@@ -201,14 +195,7 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
           const R fx = bperm_r<R>(src, cf[st].x);
           const R fy = bperm_r<R>(src, cf[st].y);
           const R tt = wt == 0 ? f0 : f1;
-          R v;
-          if constexpr (sizeof(R) == 4) {
-            v = coef[kPolyDeg];
-#pragma unroll
-            for (int k = kPolyDeg - 1; k >= 0; k--) v = fmaf(v, tt, coef[k]);
-          } else {
-            v = kb_psi<R>(tt + wm - R(ww), wm, wb, wden);
-          }
+          const R v = tc.at(tt);
           // dimension-0 lanes carry psi0 f_j, dimension-1 lanes psi1
           const R vr = wt == 0 ? v * fx : v;
           const R vi = wt == 0 ? v * fy : v;
