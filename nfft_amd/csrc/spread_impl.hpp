@@ -173,15 +173,31 @@ __device__ __forceinline__ void node_window(const int *__restrict__ cell,
                                             const R *__restrict__ frac, long p, long M,
                                             const WinParams<R> &wp, int (&c)[D],
                                             R (&psi)[D][W]) {
+  static_assert(W % 2 == 0, "W = 2m+2 taps");
+  // per-tap polynomials in tt = frac - 1/2, coefficients uniform over the
+  // wavefront (scalar loads); the table is mirror-symmetric (tap W-1-w is
+  // tap w at -tt), so a tap pair comes from tap w's even and odd parts,
+  // E(tt^2) +- tt O(tt^2): PD + 2 FMAs per pair instead of 2 PD
+  constexpr int PD = poly_deg_v<R>;
+  constexpr int KE = PD - (PD % 2);      // highest even power
+  constexpr int KO = PD - 1 + (PD % 2);  // highest odd power
 #pragma unroll
   for (int t = 0; t < D; t++) {
     c[t] = cell[t * M + p];
-    const R fr = frac[t * M + p];
-    // per-tap polynomials in (frac - 1/2); coefficients are uniform over the
-    // wavefront and come from scalar loads
-    constexpr int PD = poly_deg_v<R>;
+    const R tt = frac[t * M + p] - R(0.5);
+    const R t2 = tt * tt;
 #pragma unroll
-    for (int w = 0; w < W; w++) psi[t][w] = tap_poly<R, PD>(wp.poly + (t * W + w) * (PD + 1), fr);
+    for (int w = 0; w < W / 2; w++) {
+      const R *pc = wp.poly + (t * W + w) * (PD + 1);
+      R e = pc[KE];
+#pragma unroll
+      for (int k = KE - 2; k >= 0; k -= 2) e = fma(e, t2, pc[k]);
+      R o = pc[KO];
+#pragma unroll
+      for (int k = KO - 2; k >= 1; k -= 2) o = fma(o, t2, pc[k]);
+      psi[t][w] = fma(tt, o, e);
+      psi[t][W - 1 - w] = fma(-tt, o, e);
+    }
   }
 }
 
