@@ -8,17 +8,20 @@
 // read-modify-write per run instead of one per node.  Dense sampling (the
 // centre of radial trajectories) puts several nodes in most occupied cells.
 //
-// Tile ownership as in the row-partitioned kernel: NW wavefronts share one
-// LDS tile and wavefront w updates only tile rows e0 with e0 % NW == w, so
-// plain FP32 read-modify-write is race-free (no ds_add_f32, which is slow on
-// this GPU).  A lane owns one tap of dimension 1 and a pair of adjacent taps
-// of dimension 2; lanes past the last pair are masked off.
+// NW wavefronts share one LDS tile and wavefront w updates only tile cells
+// whose dimension-1 index e1 has e1 % NW == w (W / NW of a node's window
+// rows), so plain FP32 read-modify-write is race-free (no ds_add_f32, which
+// is slow on this GPU).  A lane owns one tap of dimension 0 and a pair of
+// adjacent taps of dimension 2; lanes past the last pair are masked off.
+// (Three wavefronts per tile, the rows split 4/3/3 with the missing row
+// masked, fit 150 VGPRs and three waves per SIMD but repeat the window work
+// once more per node: B^T 4.03 -> 4.80 ms on the 3-D kooshball.)
 //
 // Window values: G nodes at a time, each lane evaluates one tap (dimension
 // wt, tap ww) of one node as a per-tap polynomial (two nodes per pass when
 // 3 W <= 32), with the node's fractional coordinates and value fetched by
 // ds_bpermute from the lane that loaded the node.  A lane's pair weights
-// psi1 * psi2 are fetched the same way; the row weights psi0 * f_j of each
+// psi0 * psi2 are fetched the same way; the row weights psi1 * f_j of each
 // node are gathered once by ds_bpermute into lanes 0..RW-1 of every 16-lane
 // row and the FMAs read them with a DPP row broadcast.
 #pragma once
