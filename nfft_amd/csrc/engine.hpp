@@ -42,6 +42,9 @@ struct BinGeom {
   // batched launches (blockIdx.y = right-hand side): per right-hand side
   // offsets of the grid, the node vector and the scratch tiles; 0 otherwise
   long bs_g, bs_f, bs_t;
+  // right-hand sides of a batched launch whose workgroups each take several
+  // (the 2-D multi-right-hand-side spreading kernel); set at its launch
+  int nrhs;
   // interpolation slots (see nodes.hip): slots[k] = (first node, count) of
   // up to slot_nodes consecutive nodes of one cell; islots[item] = (first
   // slot, slot count); null when the plan's interpolation kernel does not
@@ -87,6 +90,10 @@ template <typename R> constexpr int poly_deg_v = sizeof(R) == 4 ? kPolyDeg : kPo
 // spreading runs of kRunCells cells; wider windows use pairs and one-cell runs
 constexpr int kSlotQuadMaxW = 10;
 constexpr int kRunCells = 3;
+// right-hand sides per workgroup of the batched 2-D run-merging spreading
+// (32 coils on 256^2: two 0.92 ms per batched trafo + adjoint, four 0.97,
+// eight 1.30, one per workgroup 1.16)
+constexpr int kRhsPerGroup = 2;
 constexpr int kPolyCoef = kPolyDeg + 1;
 
 // Twiddles and radix schedule of the FFT along one grid dimension.

@@ -1616,6 +1616,7 @@ template <typename R> struct BKernelTable {
   fn spread_runs_2sbc1[kMaxStaticM + 1];
   fn spread_runs_2rs[kMaxStaticM + 1];
   fn spread_runs2d[kMaxStaticM + 1];          // d = 2: run-merging, one wavefront
+  fn spread_runs2d_mc[kMaxStaticM + 1];       // same, kRhsPerGroup right-hand sides each
   fn interp_rt[kMaxDim][2];  // [d-1][tile]
   fn spread_rt[kMaxDim][2];
 };
@@ -1640,6 +1641,8 @@ void bk_fill_one(BKernelTable<R> &k) {
     constexpr int G2 = std::is_same_v<R, float> ? ((2 * M_ + 2) * (M_ + 1) > 64 ? 4 : 8)
                                                : ((2 * M_ + 2) * (M_ + 1) > 64 ? 2 : 4);
     k.spread_runs2d[M_] = k_spread_runs2d<R, 2 * M_ + 2, G2, 2>;
+    if constexpr (std::is_same_v<R, float>)
+      k.spread_runs2d_mc[M_] = k_spread_runs2d_mc<2 * M_ + 2, G2, 2, kRhsPerGroup>;
   }
   // (the row weights of a wavefront's W/2 rows fit one 16-lane DPP row)
   if constexpr (std::is_same_v<R, float> && D == 3 && D * (2 * M_ + 2) <= 64 && M_ + 1 <= 16) {
@@ -1771,6 +1774,18 @@ void launch_spread(const cplx<R> *f, cplx<R> *g, const Engine<R> &e, hipStream_t
   auto fn = pick_bkernel<R>(bkernels<R>(), gg, true, e.spread_variant, lds, threads);
   // the scratch reduction needs an LDS tile; the global variant adds directly
   if (!tiles) tiles = (lds > 0 && e.tile_scratch) ? e.tiles : nullptr;
+  if (nb > 1 && gg.d == 2 && e.spread_variant == 14 && tiles && gg.zoned &&
+      lds * kRhsPerGroup <= 64 * 1024 && bkernels<R>().spread_runs2d_mc[gg.m]) {
+    // 2-D batches: kRhsPerGroup right-hand sides per workgroup share the
+    // node work (scratch tiles per right-hand side as in the single launch)
+    BinGeom gm = gg;
+    gm.nrhs = nb;
+    hipLaunchKernelGGL(bkernels<R>().spread_runs2d_mc[gg.m],
+                       dim3((unsigned)e.sp_nsub, (unsigned)((nb + kRhsPerGroup - 1) / kRhsPerGroup)),
+                       dim3(threads), lds * kRhsPerGroup, s, f, g, e.perm, e.cell, e.frac,
+                       e.sp_subs, gm, e.win, tiles);
+    return;
+  }
   hipLaunchKernelGGL(fn, dim3((unsigned)e.sp_nsub, (unsigned)nb), dim3(threads), lds, s, f, g,
                      e.perm, e.cell, e.frac, e.sp_subs, gg, e.win, tiles);
 }

@@ -249,4 +249,241 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
   flush_tile<R, 2>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, sp.w, lane, 64);
 }
 
+// The same B^T for NC right-hand sides that share one node set (a batch of
+// coils): workgroup (item, y) spreads right-hand sides NC y .. NC y + NC - 1
+// of the launch into NC LDS tiles.  What depends only on the nodes is done
+// once for all of them: the coordinate fetches, the window polynomials, the
+// pair weights psi0, psi1 and the run keys; per right-hand side a node
+// costs its value (two v_readlane from the lanes that loaded it), two
+// products psi0 f_j and the 4 FMAs per pair.  Each right-hand side's sums
+// are formed exactly as in k_spread_runs2d (psi0 f_j rounded, times psi1,
+// added in node order), so the results are bit-identical to it.
+template <int W, int G, int S, int NC>
+__global__ __launch_bounds__(64) void k_spread_runs2d_mc(const float2 *__restrict__ fin,
+                                                         float2 *__restrict__ g,
+                                                         const int *__restrict__ perm,
+                                                         const int *__restrict__ cell,
+                                                         const float *__restrict__ frac,
+                                                         const int4 *__restrict__ subs,
+                                                         BinGeom gg, WinParams<float> wp,
+                                                         float2 *__restrict__ tiles) {
+  static_assert(W % 2 == 0 && 2 * W <= 64, "W = 2m+2 taps");
+  constexpr int HW = W / 2;
+  constexpr int PAIRS = W * HW;
+  constexpr int SLOTS = (PAIRS + 63) / 64;
+  constexpr int NP = 2 * W <= 32 ? 2 : 1;  // nodes per window pass
+  constexpr int NPASS = G / NP;
+  static_assert(G % NP == 0 && 64 % G == 0, "groups tile the 64-node chunk");
+  // right-hand sides of this workgroup: c0 .. c0 + nrhs - 1
+  const int c0 = blockIdx.y * NC;
+  const int nrhs = min(NC, gg.nrhs - c0);
+  fin += (long)c0 * gg.bs_f;
+  g += (long)c0 * gg.bs_g;
+  tiles += (long)c0 * gg.bs_t;
+  extern __shared__ __align__(16) unsigned char b_smem[];
+  float2 *tile = reinterpret_cast<float2 *>(b_smem);  // NC tiles of tot cells
+  const int4 sp = subs[blockIdx.x];
+  int lo[2];
+  bin_corner<2>(sp.x, gg, lo);
+  const int E1 = gg.E[1];
+  const int tot = gg.E[0] * E1;
+  const int lane = threadIdx.x;
+  for (int q = lane; q < NC * tot; q += 64) tile[q] = float2{0.f, 0.f};
+
+  // (dimension-0 tap, dimension-1 pair) slots of this lane
+  int off[SLOTS];
+  bool pv[SLOTS];
+  int adr_u[NP][SLOTS], adr_v0[NP][SLOTS], adr_v1[NP][SLOTS];
+#pragma unroll
+  for (int sl = 0; sl < SLOTS; sl++) {
+    const int q = lane + 64 * sl;
+    pv[sl] = q < PAIRS;
+    const int pa = pv[sl] ? q / HW : 0;
+    const int pb = pv[sl] ? 2 * (q - (q / HW) * HW) : 0;
+    off[sl] = pa * E1 + pb;
+#pragma unroll
+    for (int h = 0; h < NP; h++) {
+      adr_u[h][sl] = 4 * (h * 32 + pa);
+      adr_v0[h][sl] = 4 * (h * 32 + W + pb);
+      adr_v1[h][sl] = 4 * (h * 32 + W + pb + 1);
+    }
+  }
+  // window tap evaluated by this lane in a pass: node half h, dimension wt,
+  // tap ww, its polynomial in registers
+  const int h = NP == 2 ? lane >> 5 : 0;
+  const int li = NP == 2 ? lane & 31 : lane;
+  const int wt = li / W < 2 ? li / W : 1;
+  const int ww = li - (li / W) * W;
+  TapCoef<float, kPolyDeg> tc;
+  tc.load(wp.poly + (wt * W + ww) * kPolyCoef);
+
+  int sb[S], sn[S];
+  {
+    const int part = (sp.z + S - 1) / S;
+#pragma unroll
+    for (int st = 0; st < S; st++) {
+      sb[st] = min(sp.z, st * part);
+      sn[st] = min(sp.z, (st + 1) * part) - sb[st];
+    }
+  }
+  float2 acc[S][NC][SLOTS][2];
+#pragma unroll
+  for (int st = 0; st < S; st++)
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+      for (int sl = 0; sl < SLOTS; sl++) acc[st][c][sl][0] = acc[st][c][sl][1] = float2{0.f, 0.f};
+  int cur[S];
+  __syncthreads();
+  auto flush = [&](int st) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      if (c < nrhs) {
+        float2 *tc_ = tile + c * tot + cur[st];
+        float2 v[SLOTS][2];
+#pragma unroll
+        for (int sl = 0; sl < SLOTS; sl++)
+          if (pv[sl]) {
+            v[sl][0] = tc_[off[sl]];
+            v[sl][1] = tc_[off[sl] + 1];
+          }
+#pragma unroll
+        for (int sl = 0; sl < SLOTS; sl++)
+          if (pv[sl]) {
+            tc_[off[sl]] = float2{v[sl][0].x + acc[st][c][sl][0].x, v[sl][0].y + acc[st][c][sl][0].y};
+            tc_[off[sl] + 1] =
+                float2{v[sl][1].x + acc[st][c][sl][1].x, v[sl][1].y + acc[st][c][sl][1].y};
+          }
+      }
+#pragma unroll
+      for (int sl = 0; sl < SLOTS; sl++) acc[st][c][sl][0] = acc[st][c][sl][1] = float2{0.f, 0.f};
+    }
+  };
+
+  // chunk of 64 nodes per stream, one per lane (lanes past the stream's end
+  // repeat its last node with value zero; an empty stream reads the item's
+  // first node)
+  auto pos = [&](int st, int first) {
+    const int i = first + lane;
+    return (long)sp.y + (sn[st] > 0 ? sb[st] + (i < sn[st] ? i : sn[st] - 1) : 0);
+  };
+  int ckey[S];
+  float cfr[S][2];
+  float2 cf[S][NC];
+  int nc_[S][2];
+  float nfr[S][2];
+  float2 nf[S][NC];
+  int nperm[S], nnperm[S];
+  auto load_raw = [&](int st, int first, int pidx) {
+    const long p = pos(st, first);
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      nc_[st][t] = cell[t * gg.M + p];
+      nfr[st][t] = frac[t * gg.M + p];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+      nf[st][c] = c < nrhs ? fin[(long)c * gg.bs_f + pidx] : float2{0.f, 0.f};
+  };
+  auto convert = [&](int st, int first) {
+    ckey[st] = (nc_[st][0] - lo[0]) * E1 + (nc_[st][1] - lo[1]);
+#pragma unroll
+    for (int t = 0; t < 2; t++) cfr[st][t] = nfr[st][t] - 0.5f;
+#pragma unroll
+    for (int c = 0; c < NC; c++) cf[st][c] = first + lane < sn[st] ? nf[st][c] : float2{0.f, 0.f};
+  };
+  auto rl = [](float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+  };
+  int longest = 0;
+#pragma unroll
+  for (int st = 0; st < S; st++) {
+    load_raw(st, 0, perm[pos(st, 0)]);
+    nperm[st] = perm[pos(st, 64)];
+    nnperm[st] = 0;
+    convert(st, 0);
+    cur[st] = __builtin_amdgcn_readfirstlane(ckey[st]);
+    longest = max(longest, sn[st]);
+  }
+  for (int first = 0; first < longest; first += 64) {
+    int cmax = 0;
+#pragma unroll
+    for (int st = 0; st < S; st++) {
+      if (first + 64 < sn[st]) {
+        load_raw(st, first + 64, nperm[st]);
+        nnperm[st] = perm[pos(st, first + 128)];
+      }
+      cmax = max(cmax, max(0, min(64, sn[st] - first)));
+    }
+    for (int base = 0; base < cmax; base += G) {
+      // window factors of nodes base .. base+G-1 of every stream
+      float u[S][G][SLOTS], v0[S][G][SLOTS], v1[S][G][SLOTS];
+#pragma unroll
+      for (int st = 0; st < S; st++) {
+#pragma unroll
+        for (int p = 0; p < NPASS; p++) {
+          const int src = 4 * (base + NP * p + h);
+          const float f0 = bperm_r<float>(src, cfr[st][0]);
+          const float f1 = bperm_r<float>(src, cfr[st][1]);
+          const float v = tc.at(wt == 0 ? f0 : f1);
+#pragma unroll
+          for (int hk = 0; hk < NP; hk++) {
+            const int k = NP * p + hk;
+#pragma unroll
+            for (int sl = 0; sl < SLOTS; sl++) {
+              u[st][k][sl] = bperm_r<float>(adr_u[hk][sl], v);
+              v0[st][k][sl] = bperm_r<float>(adr_v0[hk][sl], v);
+              v1[st][k][sl] = bperm_r<float>(adr_v1[hk][sl], v);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < G; k++) {
+#pragma unroll
+        for (int st = 0; st < S; st++) {
+          const int l = base + k;
+          const int key = __builtin_amdgcn_readlane(ckey[st], l);
+          if (__builtin_expect(key != cur[st], 0)) {
+            flush(st);
+            cur[st] = key;
+          }
+#pragma unroll
+          for (int c = 0; c < NC; c++) {
+            const float fx = rl(cf[st][c].x, l), fy = rl(cf[st][c].y, l);
+#pragma unroll
+            for (int sl = 0; sl < SLOTS; sl++) {
+              const float ur = u[st][k][sl] * fx, ui = u[st][k][sl] * fy;
+              acc[st][c][sl][0].x = fmaf(ur, v0[st][k][sl], acc[st][c][sl][0].x);
+              acc[st][c][sl][0].y = fmaf(ui, v0[st][k][sl], acc[st][c][sl][0].y);
+              acc[st][c][sl][1].x = fmaf(ur, v1[st][k][sl], acc[st][c][sl][1].x);
+              acc[st][c][sl][1].y = fmaf(ui, v1[st][k][sl], acc[st][c][sl][1].y);
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int st = 0; st < S; st++) {
+      if (first + 64 < sn[st]) {
+        convert(st, first + 64);
+        nperm[st] = nnperm[st];
+      } else {
+        // stream done: zero-valued copies of its last node (its open run)
+        ckey[st] = __builtin_amdgcn_readlane(ckey[st], 63);
+#pragma unroll
+        for (int c = 0; c < NC; c++) cf[st][c] = float2{0.f, 0.f};
+      }
+    }
+  }
+#pragma unroll
+  for (int st = 0; st < S; st++) flush(st);
+  __syncthreads();
+  for (int c = 0; c < nrhs; c++) {
+    const float2 *tc_ = tile + c * tot;
+    flush_tile<float, 2>([&](int q) { return tc_[q]; }, tiles + (long)c * gg.bs_t,
+                         g + (long)c * gg.bs_g, lo, gg, tot, sp.w, lane, 64);
+  }
+}
+
 }  // namespace nfft_hip
