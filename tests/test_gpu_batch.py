@@ -73,24 +73,30 @@ def test_batch_matches_single_calls(precision, N, gen):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("m,nrhs", [(2, 2), (4, 8), (4, 9), (6, 3), (8, 5)])
-def test_2d_adjoint_batch_several_rhs_per_workgroup(m, nrhs):
-    """2-D FP32 batched B^T spreads two right-hand sides per workgroup
-    (shared node work, one LDS tile each; odd counts end with a group of
-    one): every right-hand side bit-identical to its single call, for
-    windows of 6 to 18 taps (one or two lane slots)."""
+def test_2d_batch_several_rhs_per_workgroup(m, nrhs):
+    """2-D FP32 batched B^T takes two right-hand sides per workgroup (shared
+    node work, one LDS tile each; odd counts end with a group of one), B one
+    per workgroup: every right-hand side bit-identical to its single call,
+    for windows of 6 to 18 taps (one or two lane slots)."""
     x = W.golden_radial(60, 64)
     N = [32, 32]
     p = plan(N, x, "single", m=m)
     try:
         NN, M = int(np.prod(N)), p.M
+        fh = rand_dev(nrhs * NN, 4, "single")
+        f = torch.empty(nrhs * M, dtype=torch.complex64, device="cuda")
+        p.trafo_batch_device(fh.data_ptr(), f.data_ptr(), nrhs)
         fin = rand_dev(nrhs * M, 5, "single")
         fho = torch.empty(nrhs * NN, dtype=torch.complex64, device="cuda")
         p.adjoint_batch_device(fin.data_ptr(), fho.data_ptr(), nrhs)
         p.synchronize()
         for r in range(nrhs):
+            one = torch.empty(M, dtype=torch.complex64, device="cuda")
+            p.trafo_device(fh[r * NN:].data_ptr(), one.data_ptr())
             oneh = torch.empty(NN, dtype=torch.complex64, device="cuda")
             p.adjoint_device(fin[r * M:].data_ptr(), oneh.data_ptr())
             p.synchronize()
+            assert torch.equal(one, f[r * M:(r + 1) * M]), r
             assert torch.equal(oneh, fho[r * NN:(r + 1) * NN]), r
     finally:
         p.finalize()
