@@ -339,19 +339,19 @@ __global__ __launch_bounds__(64) void k_spread_runs2d_mc(const float2 *__restric
 #pragma unroll
     for (int c = 0; c < NC; c++) {
       if (c < nrhs) {
-        float2 *tc_ = tile + c * tot + cur[st];
+        float2 *tr = tile + c * tot + cur[st];
         float2 v[SLOTS][2];
 #pragma unroll
         for (int sl = 0; sl < SLOTS; sl++)
           if (pv[sl]) {
-            v[sl][0] = tc_[off[sl]];
-            v[sl][1] = tc_[off[sl] + 1];
+            v[sl][0] = tr[off[sl]];
+            v[sl][1] = tr[off[sl] + 1];
           }
 #pragma unroll
         for (int sl = 0; sl < SLOTS; sl++)
           if (pv[sl]) {
-            tc_[off[sl]] = float2{v[sl][0].x + acc[st][c][sl][0].x, v[sl][0].y + acc[st][c][sl][0].y};
-            tc_[off[sl] + 1] =
+            tr[off[sl]] = float2{v[sl][0].x + acc[st][c][sl][0].x, v[sl][0].y + acc[st][c][sl][0].y};
+            tr[off[sl] + 1] =
                 float2{v[sl][1].x + acc[st][c][sl][1].x, v[sl][1].y + acc[st][c][sl][1].y};
           }
       }
@@ -480,8 +480,8 @@ __global__ __launch_bounds__(64) void k_spread_runs2d_mc(const float2 *__restric
   for (int st = 0; st < S; st++) flush(st);
   __syncthreads();
   for (int c = 0; c < nrhs; c++) {
-    const float2 *tc_ = tile + c * tot;
-    flush_tile<float, 2>([&](int q) { return tc_[q]; }, tiles + (long)c * gg.bs_t,
+    const float2 *tr = tile + c * tot;
+    flush_tile<float, 2>([&](int q) { return tr[q]; }, tiles + (long)c * gg.bs_t,
                          g + (long)c * gg.bs_g, lo, gg, tot, sp.w, lane, 64);
   }
 }
