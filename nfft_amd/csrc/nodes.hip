@@ -9,13 +9,18 @@
 // follows from c.  Instead of the reference's radix sort of linearised cell
 // keys, nodes are counting-sorted by a bin-major cell key
 //   key = bin * (bs_0 ... bs_{d-1}) + (cell offset inside the bin, row-major)
-// (histogram with atomics, multi-block exclusive scan, placement), so each bin
-// is one contiguous range and, inside it, nodes of one cell are adjacent (the
-// spreading kernels sum such runs in registers).  Every bin is cut into work
-// items of at most max_per_sub nodes for the LDS-tiled B kernels.  Order
-// inside a cell is not specified; results do not depend on it beyond
-// rounding.
+// (histogram with atomics and multi-block exclusive scan for the bin and cell
+// offsets; the order itself from a stable LSD radix sort of the keys with the
+// node indices, rocPRIM's radix_sort_pairs), so each bin is one contiguous
+// range and, inside it, nodes of one cell are adjacent (the spreading kernels
+// sum such runs in registers) in increasing node index, as the reference's
+// stable radix sort leaves them.  The order, and with it every FP32 sum, is
+// therefore the same on every run.  Every bin is cut into work items of at
+// most max_per_sub nodes for the LDS-tiled B kernels.
 #include <cstdio>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include "engine.hpp"
 
@@ -24,8 +29,7 @@ namespace nfft_hip {
 template <typename R>
 __global__ void k_node_prep(const R *__restrict__ x, BinGeom gg, int cap,
                             int *__restrict__ cell_tmp, R *__restrict__ frac_tmp,
-                            int *__restrict__ key_of, int *__restrict__ rank_of,
-                            int *__restrict__ khist) {
+                            int *__restrict__ key_of, int *__restrict__ khist) {
   long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= gg.M) return;
   int bin = 0, local = 0;
@@ -43,7 +47,7 @@ __global__ void k_node_prep(const R *__restrict__ x, BinGeom gg, int cap,
   }
   const int key = bin * cap + local;
   key_of[j] = key;
-  rank_of[j] = atomicAdd(&khist[key], 1);
+  atomicAdd(&khist[key], 1);
 }
 
 // Exclusive scan of `len` ints by one workgroup of 1024 threads; total in *sum.
@@ -142,14 +146,14 @@ __global__ void k_bins_from_keys(const int *__restrict__ kstart, long nbins, int
   hist[b] = next - first;
 }
 
+// sorted position p holds node order[p]: its index and its cell / frac
 template <typename R>
 __global__ void k_node_place(BinGeom gg, const int *__restrict__ cell_tmp,
-                             const R *__restrict__ frac_tmp, const int *__restrict__ key_of,
-                             const int *__restrict__ rank_of, const int *__restrict__ kstart,
+                             const R *__restrict__ frac_tmp, const int *__restrict__ order,
                              int *__restrict__ cell, R *__restrict__ frac, int *__restrict__ perm) {
-  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= gg.M) return;
-  long p = (long)kstart[key_of[j]] + rank_of[j];
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= gg.M) return;
+  const long j = order[p];
   perm[p] = (int)j;
   for (int t = 0; t < gg.d; t++) {
     cell[t * gg.M + p] = cell_tmp[t * gg.M + j];
@@ -412,7 +416,16 @@ void bin_nodes(Engine<R> &e) {
   const long b_M = al(M * (long)sizeof(int));
   const long b_cell = al((long)gg.d * M * (long)sizeof(int));
   const long b_frac = al((long)gg.d * M * (long)sizeof(R));
-  const long need = 2 * b_keys + 2 * b_tiles + 4 * b_bins + 2 * b_M + b_cell + b_frac + 1024;
+  // the stable node order: keys sorted with the node indices (LSD radix
+  // sort over the key's bits); its temporary storage comes from the same
+  // scratch
+  const unsigned key_bits = nkeys > 1 ? 32u - (unsigned)__builtin_clz((unsigned)(nkeys - 1)) : 1u;
+  size_t sort_bytes = 0;
+  NFFT_HIP_CHECK(R, rocprim::radix_sort_pairs(nullptr, sort_bytes, (const unsigned *)nullptr,
+                                              (unsigned *)nullptr, rocprim::counting_iterator<int>(0),
+                                              (int *)nullptr, (size_t)M, 0u, key_bits, e.stream));
+  const long b_sort = al((long)sort_bytes + 1);
+  const long need = 2 * b_keys + 2 * b_tiles + 4 * b_bins + 4 * b_M + b_cell + b_frac + b_sort + 1024;
   if (need > e.scratch_cap) {
     if (e.scratch_i) NFFT_HIP_CHECK(R, hipFree(e.scratch_i));
     NFFT_HIP_CHECK(R, hipMalloc(&e.scratch_i, need));
@@ -434,6 +447,9 @@ void bin_nodes(Engine<R> &e) {
   int *sub_start = reinterpret_cast<int *>(take(b_bins));
   int *key_of = reinterpret_cast<int *>(take(b_M));
   int *rank_of = reinterpret_cast<int *>(take(b_M));
+  unsigned *key_sorted = reinterpret_cast<unsigned *>(take(b_M));
+  int *order = reinterpret_cast<int *>(take(b_M));
+  void *sort_tmp = take(b_sort);
   int *cell_tmp = reinterpret_cast<int *>(take(b_cell));
   R *frac_tmp = reinterpret_cast<R *>(take(b_frac));
   int *sums = reinterpret_cast<int *>(take(1024));
@@ -442,7 +458,7 @@ void bin_nodes(Engine<R> &e) {
   launch_fill_zero_int(khist, nkeys, s);
   if (M > 0)
     hipLaunchKernelGGL(k_node_prep<R>, dim3((unsigned)ceil_div(M, 256)), dim3(256), 0, s, e.xdev,
-                       gg, cells_per_bin, cell_tmp, frac_tmp, key_of, rank_of, khist);
+                       gg, cells_per_bin, cell_tmp, frac_tmp, key_of, khist);
   hipLaunchKernelGGL(k_scan_tile_sums, dim3((unsigned)ntiles), dim3(256), 0, s, khist, nkeys,
                      tile_sum);
   hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, tile_sum, tile_off, ntiles,
@@ -451,9 +467,13 @@ void bin_nodes(Engine<R> &e) {
                      nkeys, tile_off);
   hipLaunchKernelGGL(k_bins_from_keys, dim3((unsigned)ceil_div(nbins, 256)), dim3(256), 0, s,
                      kstart, nbins, cells_per_bin, (int)M, hist, bin_start);
-  if (M > 0)
+  if (M > 0) {
+    NFFT_HIP_CHECK(R, rocprim::radix_sort_pairs(sort_tmp, sort_bytes, (const unsigned *)key_of,
+                                                key_sorted, rocprim::counting_iterator<int>(0),
+                                                order, (size_t)M, 0u, key_bits, s));
     hipLaunchKernelGGL(k_node_place<R>, dim3((unsigned)ceil_div(M, 256)), dim3(256), 0, s, gg,
-                       cell_tmp, frac_tmp, key_of, rank_of, kstart, e.cell, e.frac, e.perm);
+                       cell_tmp, frac_tmp, order, e.cell, e.frac, e.perm);
+  }
   // work items of interpolation (small, for balance) and of spreading
   // (larger, to bound the number of tiles written to scratch); the spreading
   // list keeps its per-bin first item and count for the tile reduction

@@ -374,3 +374,31 @@ def test_interp_pairs_on_and_off(pairs, slot, order, pitch, plane, monkeypatch):
     f = W.complex_normal(x.shape[0], seed=82)
     (fg, fr, _), (hg, hr, _) = run_pair(N, x, fh, f, 4, "single")
     assert rel_errs(fg, fr)[0] < 1e-5 and rel_errs(hg, hr)[0] < 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,gen", [([64, 64], lambda: W.clustered_2d(20000, seed=71)),
+                                   ([16, 16, 16], lambda: W.kooshball(256, 96, seed=72))],
+                         ids=["2d_clustered", "3d_kooshball"])
+def test_fp32_results_reproducible(N, gen):
+    """The node binning orders each cell's nodes by node index (a stable
+    radix sort, as the reference's), so a fresh plan on the same nodes runs
+    the same FP32 sums: trafo and adjoint bit-identical across plans (the
+    FP32 kernels add in a fixed order, with no atomics)."""
+    x = gen()
+    fh = W.complex_normal(int(np.prod(N)), seed=73)
+    f = W.complex_normal(x.shape[0], seed=74)
+    outs = []
+    for _ in range(2):
+        p = make_plan(N, x, 4, "single")
+        try:
+            p.f_hat[:] = fh
+            p.trafo()
+            ft = p.f.copy()
+            p.f[:] = f
+            p.adjoint()
+            outs.append((ft, p.f_hat.copy()))
+        finally:
+            p.finalize()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
