@@ -930,15 +930,22 @@ void Engine<R>::autotune() {
   hipEvent_t a, b;
   NFFT_HIP_CHECK(R, hipEventCreate(&a));
   NFFT_HIP_CHECK(R, hipEventCreate(&b));
+  // a candidate's time: the best of three timings of two runs each, after
+  // a warm-up run (one timing of small plans swung by up to a third and
+  // could pick a slower candidate)
   auto time_of = [&](auto &&run) {
     run();  // warm-up
-    NFFT_HIP_CHECK(R, hipEventRecord(a, stream));
-    for (int rep = 0; rep < 2; rep++) run();
-    NFFT_HIP_CHECK(R, hipEventRecord(b, stream));
-    NFFT_HIP_CHECK(R, hipEventSynchronize(b));
-    float ms = 0.f;
-    NFFT_HIP_CHECK(R, hipEventElapsedTime(&ms, a, b));
-    return ms;
+    float best = 1e30f;
+    for (int trial = 0; trial < 3; trial++) {
+      NFFT_HIP_CHECK(R, hipEventRecord(a, stream));
+      for (int rep = 0; rep < 2; rep++) run();
+      NFFT_HIP_CHECK(R, hipEventRecord(b, stream));
+      NFFT_HIP_CHECK(R, hipEventSynchronize(b));
+      float ms = 0.f;
+      NFFT_HIP_CHECK(R, hipEventElapsedTime(&ms, a, b));
+      best = std::min(best, ms);
+    }
+    return best;
   };
   // inputs: the device image of f (its values do not change the timing)
   // bin edges: the default, and the largest edge halved; in 1-D and 2-D also
