@@ -94,6 +94,9 @@ constexpr int kRunCells = 3;
 // (32 coils on 256^2: two 0.92 ms per batched trafo + adjoint, four 0.97,
 // eight 1.30, one per workgroup 1.16)
 constexpr int kRhsPerGroup = 2;
+// wavefronts per workgroup of the 2-D run-merging spreading, each on its own
+// share of the item's nodes and its own copy of the LDS tile
+constexpr int kRunWaves2d = 2;
 constexpr int kPolyCoef = kPolyDeg + 1;
 
 // Twiddles and radix schedule of the FFT along one grid dimension.

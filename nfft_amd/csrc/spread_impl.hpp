@@ -1615,7 +1615,8 @@ template <typename R> struct BKernelTable {
   fn spread_runs_2sbc[kMaxStaticM + 1];
   fn spread_runs_2sbc1[kMaxStaticM + 1];
   fn spread_runs_2rs[kMaxStaticM + 1];
-  fn spread_runs2d[kMaxStaticM + 1];          // d = 2: run-merging, one wavefront
+  fn spread_runs2d[kMaxStaticM + 1];          // d = 2: run-merging, kRunWaves2d wavefronts
+  fn spread_runs2d_1w[kMaxStaticM + 1];       // same, one wavefront (large tiles)
   fn spread_runs2d_mc[kMaxStaticM + 1];       // same, kRhsPerGroup right-hand sides each
   fn interp_rt[kMaxDim][2];  // [d-1][tile]
   fn spread_rt[kMaxDim][2];
@@ -1640,9 +1641,10 @@ void bk_fill_one(BKernelTable<R> &k) {
     // FP64 factors take twice the registers: smaller node groups
     constexpr int G2 = std::is_same_v<R, float> ? ((2 * M_ + 2) * (M_ + 1) > 64 ? 4 : 8)
                                                : ((2 * M_ + 2) * (M_ + 1) > 64 ? 2 : 4);
-    k.spread_runs2d[M_] = k_spread_runs2d<R, 2 * M_ + 2, G2, 2>;
+    k.spread_runs2d[M_] = k_spread_runs2d<R, 2 * M_ + 2, G2, 2, kRunWaves2d>;
+    k.spread_runs2d_1w[M_] = k_spread_runs2d<R, 2 * M_ + 2, G2, 2, 1>;
     if constexpr (std::is_same_v<R, float>)
-      k.spread_runs2d_mc[M_] = k_spread_runs2d_mc<2 * M_ + 2, G2, 2, kRhsPerGroup>;
+      k.spread_runs2d_mc[M_] = k_spread_runs2d_mc<2 * M_ + 2, G2, 2, kRhsPerGroup, kRunWaves2d>;
   }
   // (the row weights of a wavefront's W/2 rows fit one 16-lane DPP row)
   if constexpr (std::is_same_v<R, float> && D == 3 && D * (2 * M_ + 2) <= 64 && M_ + 1 <= 16) {
@@ -1712,7 +1714,13 @@ typename BKernelTable<R>::fn pick_bkernel(const BKernelTable<R> &k, const BinGeo
   if (tile && gg.d <= 3 && gg.m >= 1 && gg.m <= kMaxStaticM) {
     const int t = gg.d - 1;
     if (spread && variant == 14 && gg.d == 2 && k.spread_runs2d[gg.m]) {
-      threads = 64;
+      // a tile copy per wavefront while they fit 64 KB
+      if (lds * kRunWaves2d > 64 * 1024) {
+        threads = 64;
+        return k.spread_runs2d_1w[gg.m];
+      }
+      threads = 64 * kRunWaves2d;
+      lds *= kRunWaves2d;
       return k.spread_runs2d[gg.m];
     }
     if (spread && (variant == 10 || variant == 13) && gg.d == 3 && k.spread_runs_2sbc[gg.m]) {
@@ -1775,7 +1783,8 @@ void launch_spread(const cplx<R> *f, cplx<R> *g, const Engine<R> &e, hipStream_t
   // the scratch reduction needs an LDS tile; the global variant adds directly
   if (!tiles) tiles = (lds > 0 && e.tile_scratch) ? e.tiles : nullptr;
   if (nb > 1 && gg.d == 2 && e.spread_variant == 14 && tiles && gg.zoned &&
-      lds * kRhsPerGroup <= 64 * 1024 && bkernels<R>().spread_runs2d_mc[gg.m]) {
+      fn == bkernels<R>().spread_runs2d[gg.m] && lds * kRhsPerGroup <= 64 * 1024 &&
+      bkernels<R>().spread_runs2d_mc[gg.m]) {
     // 2-D batches: kRhsPerGroup right-hand sides per workgroup share the
     // node work (scratch tiles per right-hand side as in the single launch)
     BinGeom gm = gg;

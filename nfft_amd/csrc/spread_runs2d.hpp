@@ -34,8 +34,8 @@ __device__ __forceinline__ R bperm_r(int byte_addr, R v) {
   }
 }
 
-template <typename R, int W, int G, int S>
-__global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict__ fin,
+template <typename R, int W, int G, int S, int NWV>
+__global__ __launch_bounds__(64 * NWV) void k_spread_runs2d(const cplx<R> *__restrict__ fin,
                                                       cplx<R> *__restrict__ g,
                                                       const int *__restrict__ perm,
                                                       const int *__restrict__ cell,
@@ -56,14 +56,18 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
   g += (long)blockIdx.y * gg.bs_g;
   if (tiles) tiles += (long)blockIdx.y * gg.bs_t;
   extern __shared__ __align__(16) unsigned char b_smem[];
-  C *tile = reinterpret_cast<C *>(b_smem);
+  // NWV wavefronts, each with its own copy of the tile (summed when the
+  // tile is written out)
+  C *tile0 = reinterpret_cast<C *>(b_smem);
   const int4 sp = subs[blockIdx.x];
   int lo[2];
   bin_corner<2>(sp.x, gg, lo);
   const int E1 = gg.E[1];
   const int tot = gg.E[0] * E1;
-  const int lane = threadIdx.x;
-  for (int q = lane; q < tot; q += 64) tile[q] = C{R(0), R(0)};
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  C *tile = tile0 + wv * tot;
+  for (int q = threadIdx.x; q < NWV * tot; q += 64 * NWV) tile0[q] = C{R(0), R(0)};
 
   // (dimension-0 tap, dimension-1 pair) slots of this lane
   int off[SLOTS];
@@ -92,13 +96,14 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
   TapCoef<R, poly_deg_v<R>> tc;
   tc.load(wp.poly + (wt * W + ww) * (poly_deg_v<R> + 1));
 
+  // streams: parts wv S .. wv S + S - 1 of the item's NWV S node ranges
   int sb[S], sn[S];
   {
-    const int part = (sp.z + S - 1) / S;
+    const int part = (sp.z + NWV * S - 1) / (NWV * S);
 #pragma unroll
     for (int st = 0; st < S; st++) {
-      sb[st] = min(sp.z, st * part);
-      sn[st] = min(sp.z, (st + 1) * part) - sb[st];
+      sb[st] = min(sp.z, (wv * S + st) * part);
+      sn[st] = min(sp.z, (wv * S + st + 1) * part) - sb[st];
     }
   }
   C acc[S][SLOTS][2];
@@ -246,7 +251,14 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
 #pragma unroll
   for (int st = 0; st < S; st++) flush(st);
   __syncthreads();
-  flush_tile<R, 2>([&](int q) { return tile[q]; }, tiles, g, lo, gg, tot, sp.w, lane, 64);
+  flush_tile<R, 2>(
+      [&](int q) {
+        C v = tile0[q];
+#pragma unroll
+        for (int w = 1; w < NWV; w++) v = C{v.x + tile0[w * tot + q].x, v.y + tile0[w * tot + q].y};
+        return v;
+      },
+      tiles, g, lo, gg, tot, sp.w, threadIdx.x, 64 * NWV);
 }
 
 // The same B^T for NC right-hand sides that share one node set (a batch of
@@ -258,8 +270,8 @@ __global__ __launch_bounds__(64) void k_spread_runs2d(const cplx<R> *__restrict_
 // products psi0 f_j and the 4 FMAs per pair.  Each right-hand side's sums
 // are formed exactly as in k_spread_runs2d (psi0 f_j rounded, times psi1,
 // added in node order), so the results are bit-identical to it.
-template <int W, int G, int S, int NC>
-__global__ __launch_bounds__(64) void k_spread_runs2d_mc(const float2 *__restrict__ fin,
+template <int W, int G, int S, int NC, int NWV>
+__global__ __launch_bounds__(64 * NWV) void k_spread_runs2d_mc(const float2 *__restrict__ fin,
                                                          float2 *__restrict__ g,
                                                          const int *__restrict__ perm,
                                                          const int *__restrict__ cell,
@@ -281,14 +293,17 @@ __global__ __launch_bounds__(64) void k_spread_runs2d_mc(const float2 *__restric
   g += (long)c0 * gg.bs_g;
   tiles += (long)c0 * gg.bs_t;
   extern __shared__ __align__(16) unsigned char b_smem[];
-  float2 *tile = reinterpret_cast<float2 *>(b_smem);  // NC tiles of tot cells
+  // per wavefront NC tiles of tot cells (the NWV copies summed at the end)
+  float2 *tile0 = reinterpret_cast<float2 *>(b_smem);
   const int4 sp = subs[blockIdx.x];
   int lo[2];
   bin_corner<2>(sp.x, gg, lo);
   const int E1 = gg.E[1];
   const int tot = gg.E[0] * E1;
-  const int lane = threadIdx.x;
-  for (int q = lane; q < NC * tot; q += 64) tile[q] = float2{0.f, 0.f};
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  float2 *tile = tile0 + wv * NC * tot;
+  for (int q = threadIdx.x; q < NWV * NC * tot; q += 64 * NWV) tile0[q] = float2{0.f, 0.f};
 
   // (dimension-0 tap, dimension-1 pair) slots of this lane
   int off[SLOTS];
@@ -317,13 +332,14 @@ __global__ __launch_bounds__(64) void k_spread_runs2d_mc(const float2 *__restric
   TapCoef<float, kPolyDeg> tc;
   tc.load(wp.poly + (wt * W + ww) * kPolyCoef);
 
+  // streams: parts wv S .. wv S + S - 1 of the item's NWV S node ranges
   int sb[S], sn[S];
   {
-    const int part = (sp.z + S - 1) / S;
+    const int part = (sp.z + NWV * S - 1) / (NWV * S);
 #pragma unroll
     for (int st = 0; st < S; st++) {
-      sb[st] = min(sp.z, st * part);
-      sn[st] = min(sp.z, (st + 1) * part) - sb[st];
+      sb[st] = min(sp.z, (wv * S + st) * part);
+      sn[st] = min(sp.z, (wv * S + st + 1) * part) - sb[st];
     }
   }
   float2 acc[S][NC][SLOTS][2];
@@ -480,9 +496,17 @@ __global__ __launch_bounds__(64) void k_spread_runs2d_mc(const float2 *__restric
   for (int st = 0; st < S; st++) flush(st);
   __syncthreads();
   for (int c = 0; c < nrhs; c++) {
-    const float2 *tr = tile + c * tot;
-    flush_tile<float, 2>([&](int q) { return tr[q]; }, tiles + (long)c * gg.bs_t,
-                         g + (long)c * gg.bs_g, lo, gg, tot, sp.w, lane, 64);
+    const float2 *tr = tile0 + c * tot;
+    flush_tile<float, 2>(
+        [&](int q) {
+          float2 v = tr[q];
+#pragma unroll
+          for (int w = 1; w < NWV; w++)
+            v = float2{v.x + tr[w * NC * tot + q].x, v.y + tr[w * NC * tot + q].y};
+          return v;
+        },
+        tiles + (long)c * gg.bs_t, g + (long)c * gg.bs_g, lo, gg, tot, sp.w, threadIdx.x,
+        64 * NWV);
   }
 }
 
