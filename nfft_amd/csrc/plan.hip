@@ -947,6 +947,13 @@ void Engine<R>::autotune() {
     }
     return best;
   };
+  // the plan's own transform pair a few times first, so that the first
+  // candidate is not timed on a GPU still raising its clocks (an idle GPU
+  // had the default 2-D spreading kernel timed slow and a slower one picked)
+  for (int rep = 0; rep < 4; rep++) {
+    spread_to_grid(f);
+    interp_from_grid(f);
+  }
   // inputs: the device image of f (its values do not change the timing)
   // bin edges: the default, and the largest edge halved; in 1-D and 2-D also
   // every edge doubled and every edge halved; all within the LDS budget of
@@ -1083,6 +1090,13 @@ void Engine<R>::autotune() {
       }
     }
     set_shift(pick);
+  }
+  if (const char *v = std::getenv("NFFT_HIP_TUNE_LOG"); v && std::atoi(v) != 0) {
+    std::fprintf(stderr, "[nfft_hip tune] d=%d m=%d M=%ld bins", d, m, M);
+    for (int t = 0; t < d; t++) std::fprintf(stderr, "%s%d", t ? "," : " ", geom.bs[t]);
+    std::fprintf(stderr, " spread=%d item=%d interp=%d fft_shift=%d\n", spread_variant,
+                 spread_max_per_sub, interp_variant,
+                 fwd_passes.empty() ? 0 : fwd_passes[0].lpb_shift);
   }
   hipEventDestroy(a);
   hipEventDestroy(b);
