@@ -1,14 +1,16 @@
 // B^T for d = 2 (FP32 and FP64) with same-cell runs summed in registers.
 //
-// One wavefront per work item and LDS tile (E0 x E1 cells).  A lane owns one
-// tap of dimension 0 and a pair of adjacent taps of dimension 1 of the
-// current node, so its read-modify-writes of the tile never collide with
-// another lane's (W (W/2) pairs over SLOTS passes of the 64 lanes).  Nodes
-// arrive sorted by cell; consecutive nodes of one cell form a run whose
-// contributions the lanes sum in registers and add to the tile once.  Two
-// node streams (the two halves of the work item) interleave node by node for
-// two independent dependency chains; a wavefront's LDS operations complete
-// in program order, so the streams' updates of shared cells do not race.
+// NWV wavefronts per work item, each with its own copy of the LDS tile (E0 x
+// E1 cells; the copies are summed in a fixed order when the tile is written
+// out).  A lane owns one tap of dimension 0 and a pair of adjacent taps of
+// dimension 1 of the current node, so its read-modify-writes of the tile
+// never collide with another lane's (W (W/2) pairs over SLOTS passes of the
+// 64 lanes).  Nodes arrive sorted by cell; consecutive nodes of one cell form
+// a run whose contributions the lanes sum in registers and add to the tile
+// once.  The item's nodes are cut into NWV S ranges; each wavefront walks S
+// of them as node streams interleaved node by node, independent dependency
+// chains; a wavefront's LDS operations complete in program order, so its
+// streams' updates of shared cells do not race.
 //
 // Window values: G nodes at a time, each lane evaluates one tap (dimension
 // wt, tap ww) of one node, two nodes per pass when 2 W <= 32, as a per-tap
